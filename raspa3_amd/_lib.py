@@ -1,0 +1,90 @@
+"""ctypes binding of libraspa3_amd.so (the C ABI declared in include/raspa3_amd/rg.h).
+
+The library must have been built (``__graft_entry__.build()`` or ``make -C raspa3_amd``).  Loading fails with an
+error if it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libraspa3_amd.so")
+
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+RG_OK, RG_ERR_INVALID, RG_ERR_UNSUPPORTED, RG_ERR_DEVICE, RG_ERR_NOMEM = range(5)
+RG_GRID_LENNARD_JONES, RG_GRID_EWALD_REAL = 0, 3
+
+# name -> (restype, argtypes); every symbol declared in include/raspa3_amd/rg.h
+SIGNATURES = {
+    "rg_version": (C.c_char_p, []),
+    "rg_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "rg_ctx_destroy": (C.c_int, [vp]),
+    "rg_last_error": (C.c_char_p, [vp]),
+    "rg_sync": (C.c_int, [vp]),
+    "rg_stream": (vp, [vp]),
+    "rg_box": (C.c_int, [C.c_double] * 6 + [C.c_int, dp, dp, ip, dp, dp]),
+    "rg_grid_dims": (C.c_int, [dp, C.c_double, ip]),
+    "rg_forcefield_create": (C.c_int, [vp, C.c_int, dp, dp, ip, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_int, C.c_double, C.POINTER(vp)]),
+    "rg_forcefield_destroy": (C.c_int, [vp]),
+    "rg_forcefield_pairs": (C.c_int, [vp, dp, dp, dp, ip]),
+    "rg_framework_create": (C.c_int, [vp, dp, C.c_int, C.c_int, dp, ip, dp, C.POINTER(vp)]),
+    "rg_framework_destroy": (C.c_int, [vp]),
+    "rg_grid_create": (C.c_int, [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    "rg_grid_destroy": (C.c_int, [vp]),
+    "rg_grid_device_data": (vp, [vp]),
+    "rg_grid_build": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp]),
+    "rg_grid_build_many": (C.c_int, [vp, vp, vp, C.c_int, ip, ip, dp, C.POINTER(vp)]),
+    "rg_grid_build_many_async": (C.c_int, [vp, vp, vp, C.c_int, ip, ip, dp, C.POINTER(vp)]),
+    "rg_grid_download": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
+    "rg_grid_upload": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
+    "rg_grid_interpolate": (C.c_int, [vp, vp, C.c_int, dp, dp, dp]),
+}
+
+
+class RGError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RGError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C raspa3_amd)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def d(a):
+    return a.ctypes.data_as(dp) if a is not None else None
+
+
+def i(a):
+    return a.ctypes.data_as(ip) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def check(status, ctx=None, what=""):
+    if status != RG_OK:
+        msg = lib().rg_last_error(ctx).decode() if ctx else ""
+        raise RGError(f"{what}: status {status}: {msg}")

@@ -1,0 +1,760 @@
+// Tricubic interpolation-grid builder for gfx950.
+//
+// Replaces InterpolationEnergyGrid::makeFrameworkInterpolationGrid (tricubic branch,
+// src/raspakit/interpolation_energy_grid.cpp:3800-3875) and the per-point analytic sum
+// Interactions::calculateTricubicFractionalAtPosition (src/raspakit/interactions/framework_molecule_grid.cpp:168-282).
+//
+// The reference visits every supercell atom at every point and applies the minimum-image convention per pair.
+// With a supercell at least twice the VDW cutoff wide, the set of in-cutoff pairs it forms is exactly the set of
+// periodic images of the unit-cell atoms within the cutoff of the point, so this builder enumerates those images
+// once per framework, bins them in fractional space, and lets each workgroup gather only the images that can be
+// within the cutoff of its tile of grid points.
+//
+// Work decomposition: one workgroup = 256 lanes = an 8x8x4 tile of grid points, each wavefront a 4x4x4
+// sub-block, one grid point per lane.  Candidate images are staged through LDS in chunks of 256; every lane of a
+// wavefront reads the same candidate (an LDS broadcast), so the per-candidate branches (cutoff, LJ-active,
+// charged) are uniform or nearly so.  All arithmetic is FP64.  Several grids of equal dimensions (LJ grids for
+// different probe pseudo-atoms and the unit-charge Ewald real-space grid) are built in one pass and share the
+// distance work.
+//
+// Triclinic fractional derivatives use the lattice vectors h_a and the metric G_ab = h_a.h_b instead of the
+// reference's full 3x3x3 Cartesian tensors: with u_a = h_a.dr,
+//   dE/ds_a          = sum f1 u_a
+//   d2E/ds_a ds_b    = sum f2 u_a u_b + G_ab sum f1
+//   d3E/ds0 ds1 ds2  = sum f3 u0 u1 u2 + G01 sum f2 u2 + G02 sum f2 u1 + G12 sum f2 u0
+// which equals the reference's contraction (framework_molecule_grid.cpp:244-278) term by term.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "rg_internal.hpp"
+
+namespace
+{
+
+struct alignas(16) Img
+{
+  double x, y, z, q;    // Cartesian position of the image, charge (e)
+  double u0, u1, u2;    // u_a = h_a . x (lattice-vector projections, for triclinic derivatives)
+  int type;             // local framework type row
+  int pad;
+};
+static_assert(sizeof(Img) == 64, "Img must be 64 bytes");
+
+struct alignas(16) LJPar
+{
+  double e4, s2, sh, c1, c2, c3, p0, p1;  // 4eps, sigma^2, shift, -6*4eps, 24*4eps, -96*4eps
+};
+
+constexpr int TI = 8, TJ = 8, TK = 4;
+constexpr int MAXG = 4;    // LJ grids per launch
+constexpr int MAXT = 16;   // distinct framework pseudo-atom types
+constexpr int CH = 256;    // candidates per LDS chunk
+constexpr int MAXROWS = 1024;
+
+struct GridArgs
+{
+  double cell[9];
+  double delta[3];
+  double ncm1[3];
+  double G01, G02, G12;
+  double diag[3];
+  double rc2;
+  double cut2;
+  double ext[3];
+  double lo[3], bw[3];
+  int nb[3];
+  int n[3];
+  int ntile[3];
+  int ntiles;
+  const Img* img;
+  const int* bin_start;
+  const LJPar* ljpar;  // [NLJ][MAXT]
+  double alpha, alpha2, k2a;
+  double coulomb;
+  double overlap;
+  double* out[MAXG + 1];
+};
+
+__device__ __forceinline__ double rcp64(double x)
+{
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+
+__device__ __forceinline__ double rsq64(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+template <bool TRIC>
+struct Acc
+{
+  double e, d1[3], d2[3], d3;  // d2: 01, 02, 12
+  double s1, b[3];             // triclinic metric terms: sum f1, sum f2 u_a
+  __device__ void zero()
+  {
+    e = d3 = s1 = 0.0;
+    for (int a = 0; a < 3; ++a) d1[a] = d2[a] = b[a] = 0.0;
+  }
+  __device__ __forceinline__ void add(double ee, double f1, double f2, double f3, double u0, double u1, double u2,
+                                      double u01, double u02, double u12, double u012)
+  {
+    e += ee;
+    d1[0] = fma(f1, u0, d1[0]);
+    d1[1] = fma(f1, u1, d1[1]);
+    d1[2] = fma(f1, u2, d1[2]);
+    d2[0] = fma(f2, u01, d2[0]);
+    d2[1] = fma(f2, u02, d2[1]);
+    d2[2] = fma(f2, u12, d2[2]);
+    d3 = fma(f3, u012, d3);
+    if (TRIC)
+    {
+      s1 += f1;
+      b[0] = fma(f2, u0, b[0]);
+      b[1] = fma(f2, u1, b[1]);
+      b[2] = fma(f2, u2, b[2]);
+    }
+  }
+};
+
+template <int NLJ, bool COUL, bool TRIC>
+__global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
+{
+  constexpr int NG = NLJ + (COUL ? 1 : 0);
+  __shared__ Img cand[CH];
+  __shared__ LJPar par[NLJ > 0 ? NLJ : 1][MAXT];
+  __shared__ int rowStart[MAXROWS];
+  __shared__ int rowPrefix[MAXROWS + 1];
+  __shared__ int waveCount[4];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+
+  // XCD-aware tile order: hardware places workgroup b on XCD b % 8; give each XCD a contiguous range of tiles.
+  int bid = blockIdx.x;
+  {
+    const int nfull = (A.ntiles / 8) * 8;
+    if (bid < nfull) bid = (bid % 8) * (A.ntiles / 8) + bid / 8;
+  }
+  const int tx = bid % A.ntile[0];
+  const int ty = (bid / A.ntile[0]) % A.ntile[1];
+  const int tz = bid / (A.ntile[0] * A.ntile[1]);
+
+  if (NLJ > 0)
+  {
+    for (int q = tid; q < NLJ * MAXT; q += 256) (&par[0][0])[q] = A.ljpar[q];
+  }
+
+  // this lane's grid point
+  const int gi = tx * TI + (wv & 1) * 4 + (lane & 3);
+  const int gj = ty * TJ + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  const int gk = tz * TK + (lane >> 4);
+  const bool active = gi < A.n[0] && gj < A.n[1] && gk < A.n[2];
+  // s = (i*delta_x, j*delta_y, k*delta_z); pos = cell * s (src/raspakit/interpolation_energy_grid.cpp:3829-3831)
+  const double s0 = double(gi) * A.delta[0], s1 = double(gj) * A.delta[1], s2 = double(gk) * A.delta[2];
+  const double px = A.cell[0] * s0 + A.cell[3] * s1 + A.cell[6] * s2;
+  const double py = A.cell[1] * s0 + A.cell[4] * s1 + A.cell[7] * s2;
+  const double pz = A.cell[2] * s0 + A.cell[5] * s1 + A.cell[8] * s2;
+  double pu0 = 0.0, pu1 = 0.0, pu2 = 0.0;
+  if (TRIC)
+  {
+    pu0 = A.cell[0] * px + A.cell[1] * py + A.cell[2] * pz;
+    pu1 = A.cell[3] * px + A.cell[4] * py + A.cell[5] * pz;
+    pu2 = A.cell[6] * px + A.cell[7] * py + A.cell[8] * pz;
+  }
+
+  // tile centre (fractional and Cartesian) and the bin box that holds every image within rc of the tile
+  const double fc0 = (double(tx * TI) + 0.5 * (TI - 1)) * A.delta[0];
+  const double fc1 = (double(ty * TJ) + 0.5 * (TJ - 1)) * A.delta[1];
+  const double fc2 = (double(tz * TK) + 0.5 * (TK - 1)) * A.delta[2];
+  const double cxc = A.cell[0] * fc0 + A.cell[3] * fc1 + A.cell[6] * fc2;
+  const double cyc = A.cell[1] * fc0 + A.cell[4] * fc1 + A.cell[7] * fc2;
+  const double czc = A.cell[2] * fc0 + A.cell[5] * fc1 + A.cell[8] * fc2;
+  int blo[3], bhi[3];
+  {
+    const double fc[3] = {fc0, fc1, fc2};
+    for (int a = 0; a < 3; ++a)
+    {
+      int l = int(floor((fc[a] - A.ext[a] - A.lo[a]) / A.bw[a]));
+      int h = int(floor((fc[a] + A.ext[a] - A.lo[a]) / A.bw[a]));
+      blo[a] = max(0, min(A.nb[a] - 1, l));
+      bhi[a] = max(0, min(A.nb[a] - 1, h));
+    }
+  }
+  const int n1r = bhi[1] - blo[1] + 1;
+  const int nrow = n1r * (bhi[2] - blo[2] + 1);
+
+  // rows of bins contiguous along axis 0: [start(blo0, by, bz), start(bhi0 + 1, by, bz))
+  for (int r = tid; r < nrow; r += 256)
+  {
+    const int by = blo[1] + r % n1r;
+    const int bz = blo[2] + r / n1r;
+    const int base = A.nb[0] * (by + A.nb[1] * bz);
+    const int st = A.bin_start[base + blo[0]];
+    const int en = A.bin_start[base + bhi[0] + 1];
+    rowStart[r] = st;
+    rowPrefix[r + 1] = en - st;
+  }
+  if (tid == 0) rowPrefix[0] = 0;
+  __syncthreads();
+  if (wv == 0)
+  {
+    int carry = 0;
+    for (int c0 = 0; c0 < nrow; c0 += 64)
+    {
+      const int r = c0 + lane;
+      int v = (r < nrow) ? rowPrefix[r + 1] : 0;
+      for (int o = 1; o < 64; o <<= 1)
+      {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+      }
+      if (r < nrow) rowPrefix[r + 1] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+  }
+  __syncthreads();
+  const int total = rowPrefix[nrow];
+
+  Acc<TRIC> acc[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) acc[g].zero();
+
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  for (int base = 0; base < total; base += 256)
+  {
+    // ---- gather + sphere filter + compaction into LDS
+    const int g = base + tid;
+    bool pass = false;
+    Img im;
+    if (g < total)
+    {
+      int lo = 0, hi = nrow - 1;
+      while (lo < hi)
+      {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rowPrefix[mid] <= g)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      im = A.img[rowStart[lo] + (g - rowPrefix[lo])];
+      const double dx = im.x - cxc, dy = im.y - cyc, dz = im.z - czc;
+      pass = dx * dx + dy * dy + dz * dz < A.cut2;
+      bool useful = COUL && im.q != 0.0;
+#pragma unroll
+      for (int q = 0; q < NLJ; ++q) useful = useful || par[q][im.type].e4 != 0.0;
+      pass = pass && useful;
+    }
+    const unsigned long long m = __ballot(pass);
+    if (lane == 0) waveCount[wv] = __popcll(m);
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < wv; ++q) off += waveCount[q];
+    const int ncand = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+    if (pass)
+    {
+      im.q *= A.coulomb;
+      cand[off + __popcll(m & lt_mask)] = im;
+    }
+    __syncthreads();
+
+    // ---- per-lane pair sums over the staged candidates
+    for (int c = 0; c < ncand; ++c)
+    {
+      const Img& b = cand[c];
+      const double dx = px - b.x, dy = py - b.y, dz = pz - b.z;
+      double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < A.rc2)
+      {
+        rr = fmax(rr, 0.1);  // src/raspakit/interactions/framework_molecule_grid.cpp:200
+        double u0, u1, u2;
+        if (TRIC)
+        {
+          u0 = pu0 - b.u0;
+          u1 = pu1 - b.u1;
+          u2 = pu2 - b.u2;
+        }
+        else
+        {
+          u0 = dx;
+          u1 = dy;
+          u2 = dz;
+        }
+        const double u01 = u0 * u1, u02 = u0 * u2, u12 = u1 * u2, u012 = u01 * u2;
+        if (NLJ > 0)
+        {
+          const double ir2 = rcp64(rr);
+          const double ir4 = ir2 * ir2, ir6 = ir4 * ir2;
+#pragma unroll
+          for (int q = 0; q < NLJ; ++q)
+          {
+            const LJPar& P = par[q][b.type];
+            if (P.e4 != 0.0)
+            {
+              // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46
+              const double x = P.s2 * ir2;
+              const double t3 = x * x * x;
+              const double t6 = t3 * t3;
+              const double e = fma(P.e4, t6 - t3, -P.sh);
+              const double f1 = P.c1 * fma(2.0, t6, -t3) * ir2;
+              const double f2 = P.c2 * fma(7.0, t6, -2.0 * t3) * ir4;
+              const double f3 = P.c3 * fma(28.0, t6, -5.0 * t3) * ir6;
+              acc[q].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
+            }
+          }
+        }
+        if (COUL && b.q != 0.0)
+        {
+          // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 with a unit probe charge;
+          // b.q already carries the Coulomb conversion factor.
+          const double ir = rsq64(rr);
+          const double r = rr * ir;
+          const double ir2 = ir * ir;
+          const double ir3 = ir2 * ir;
+          const double ir5 = ir3 * ir2;
+          const double ir7 = ir5 * ir2;
+          const double ec = erfc(A.alpha * r);
+          const double ex = exp(-A.alpha2 * rr);
+          const double etr = A.k2a * r * ex;
+          const double ar2 = A.alpha2 * rr;
+          const double e = b.q * ec * ir;
+          const double f1 = -b.q * (etr + ec) * ir3;
+          const double f2 = b.q * fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir5;
+          const double f3 = -b.q * fma(etr, fma(ar2, fma(4.0, ar2, 10.0), 15.0), 15.0 * ec) * ir7;
+          acc[NG - 1].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (!active) return;
+  const size_t pidx = 8 * (size_t(gi) + size_t(A.n[0]) * (size_t(gj) + size_t(A.n[1]) * size_t(gk)));
+#pragma unroll
+  for (int q = 0; q < NG; ++q)
+  {
+    double v[8];
+    v[0] = acc[q].e;
+    if (TRIC)
+    {
+      v[1] = acc[q].d1[0];
+      v[2] = acc[q].d1[1];
+      v[3] = acc[q].d1[2];
+      v[4] = fma(A.G01, acc[q].s1, acc[q].d2[0]);
+      v[5] = fma(A.G02, acc[q].s1, acc[q].d2[1]);
+      v[6] = fma(A.G12, acc[q].s1, acc[q].d2[2]);
+      v[7] = acc[q].d3 + A.G01 * acc[q].b[2] + A.G02 * acc[q].b[1] + A.G12 * acc[q].b[0];
+    }
+    else
+    {
+      // src/raspakit/interactions/framework_molecule_grid.cpp:236-243
+      v[1] = acc[q].d1[0] * A.diag[0];
+      v[2] = acc[q].d1[1] * A.diag[1];
+      v[3] = acc[q].d1[2] * A.diag[2];
+      v[4] = acc[q].d2[0] * A.diag[0] * A.diag[1];
+      v[5] = acc[q].d2[1] * A.diag[0] * A.diag[2];
+      v[6] = acc[q].d2[2] * A.diag[1] * A.diag[2];
+      v[7] = acc[q].d3 * A.diag[0] * A.diag[1] * A.diag[2];
+    }
+    // src/raspakit/interpolation_energy_grid.cpp:3849-3874
+    v[1] /= A.ncm1[0];
+    v[2] /= A.ncm1[1];
+    v[3] /= A.ncm1[2];
+    v[4] /= A.ncm1[0] * A.ncm1[1];
+    v[5] /= A.ncm1[0] * A.ncm1[2];
+    v[6] /= A.ncm1[1] * A.ncm1[2];
+    v[7] /= A.ncm1[0] * A.ncm1[1] * A.ncm1[2];
+    if (v[0] > A.overlap)
+    {
+      v[0] = 2.0 * A.overlap;
+      v[1] = fmin(fmax(v[1], -A.overlap), A.overlap);
+      v[2] = fmin(fmax(v[2], -A.overlap), A.overlap);
+      v[3] = fmin(fmax(v[3], -A.overlap), A.overlap);
+      v[4] = v[5] = v[6] = v[7] = 0.0;
+    }
+    double2* o = reinterpret_cast<double2*>(A.out[q] + pidx);
+    o[0] = make_double2(v[0], v[1]);
+    o[1] = make_double2(v[2], v[3]);
+    o[2] = make_double2(v[4], v[5]);
+    o[3] = make_double2(v[6], v[7]);
+  }
+}
+
+using KernelFn = void (*)(GridArgs);
+
+template <int NLJ, bool COUL>
+KernelFn pick2(bool tric)
+{
+  return tric ? k_grid_build<NLJ, COUL, true> : k_grid_build<NLJ, COUL, false>;
+}
+
+KernelFn pick(int nlj, bool coul, bool tric)
+{
+  switch (nlj)
+  {
+    case 0: return coul ? pick2<0, true>(tric) : nullptr;
+    case 1: return coul ? pick2<1, true>(tric) : pick2<1, false>(tric);
+    case 2: return coul ? pick2<2, true>(tric) : pick2<2, false>(tric);
+    case 3: return coul ? pick2<3, true>(tric) : pick2<3, false>(tric);
+    case 4: return coul ? pick2<4, true>(tric) : pick2<4, false>(tric);
+  }
+  return nullptr;
+}
+
+// Builds (once per framework and cutoff) the binned periodic images of the unit-cell atoms that can lie within
+// rc of a point of the closed unit cell [0,1]^3.
+rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeRow)
+{
+  ImageSet& S = fw->images;
+  if (S.rc == rc && S.d_img) return RG_OK;
+  rg_ctx* ctx = fw->ctx;
+  if (S.d_img) hipFree(S.d_img);
+  if (S.d_bin_start) hipFree(S.d_bin_start);
+  S.d_img = nullptr;
+  S.d_bin_start = nullptr;
+  S.rc = -1.0;
+
+  double m[3];
+  for (int a = 0; a < 3; ++a)
+  {
+    m[a] = rc / fw->widths[a] + 1e-9;
+    S.lo[a] = -m[a];
+    const double span = 1.0 + 2.0 * m[a];
+    S.nb[a] = std::max(1, std::min(512, static_cast<int>(span * fw->widths[a] / 2.0)));
+    S.bw[a] = span / S.nb[a];
+  }
+  std::vector<Img> imgs;
+  std::vector<int> bin;
+  const double* H = fw->cell;
+  for (int at = 0; at < fw->n; ++at)
+  {
+    const double* x = &fw->pos[3 * size_t(at)];
+    double f[3];
+    for (int a = 0; a < 3; ++a) f[a] = fw->inv[a] * x[0] + fw->inv[3 + a] * x[1] + fw->inv[6 + a] * x[2];
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a)
+    {
+      lo[a] = static_cast<int>(std::ceil(-m[a] - f[a]));
+      hi[a] = static_cast<int>(std::floor(1.0 + m[a] - f[a]));
+    }
+    for (int n2 = lo[2]; n2 <= hi[2]; ++n2)
+      for (int n1 = lo[1]; n1 <= hi[1]; ++n1)
+        for (int n0 = lo[0]; n0 <= hi[0]; ++n0)
+        {
+          Img im{};
+          im.x = x[0] + n0 * H[0] + n1 * H[3] + n2 * H[6];
+          im.y = x[1] + n0 * H[1] + n1 * H[4] + n2 * H[7];
+          im.z = x[2] + n0 * H[2] + n1 * H[5] + n2 * H[8];
+          im.q = fw->charge[at];
+          im.u0 = H[0] * im.x + H[1] * im.y + H[2] * im.z;
+          im.u1 = H[3] * im.x + H[4] * im.y + H[5] * im.z;
+          im.u2 = H[6] * im.x + H[7] * im.y + H[8] * im.z;
+          im.type = typeRow[fw->type[at]];
+          const double fi[3] = {f[0] + n0, f[1] + n1, f[2] + n2};
+          int b[3];
+          for (int a = 0; a < 3; ++a)
+            b[a] = std::max(0, std::min(S.nb[a] - 1, static_cast<int>(std::floor((fi[a] - S.lo[a]) / S.bw[a]))));
+          imgs.push_back(im);
+          bin.push_back(b[0] + S.nb[0] * (b[1] + S.nb[1] * b[2]));
+        }
+  }
+  const int nbins = S.nb[0] * S.nb[1] * S.nb[2];
+  std::vector<int> start(nbins + 1, 0);
+  for (int b : bin) start[b + 1]++;
+  for (int b = 0; b < nbins; ++b) start[b + 1] += start[b];
+  std::vector<Img> sorted(imgs.size());
+  {
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (size_t q = 0; q < imgs.size(); ++q) sorted[fill[bin[q]]++] = imgs[q];
+  }
+  S.n_images = static_cast<int>(sorted.size());
+  RG_HIP(ctx, hipMalloc(&S.d_img, std::max<size_t>(1, sorted.size()) * sizeof(Img)));
+  RG_HIP(ctx, hipMalloc(&S.d_bin_start, (nbins + 1) * sizeof(int)));
+  if (!sorted.empty())
+    RG_HIP(ctx, hipMemcpyAsync(S.d_img, sorted.data(), sorted.size() * sizeof(Img), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(S.d_bin_start, start.data(), (nbins + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  S.rc = rc;
+  return RG_OK;
+}
+
+rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fwc, int n_grids,
+                        const rg_grid_type* types, const int* probes, const double* cutoffs, rg_grid* const* grids,
+                        bool sync)
+{
+  if (!ctx || !ff || !fwc || n_grids <= 0 || !types || !probes || !cutoffs || !grids)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: invalid arguments");
+  rg_framework* fw = const_cast<rg_framework*>(fwc);
+  const rg_grid* g0 = grids[0];
+  for (int g = 0; g < n_grids; ++g)
+  {
+    const rg_grid* G = grids[g];
+    if (!G || !G->d) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: grid %d is null", g);
+    if (G->n[0] != g0->n[0] || G->n[1] != g0->n[1] || G->n[2] != g0->n[2])
+      return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: fused grids must have equal dimensions");
+    if (std::memcmp(G->cell, fw->cell, sizeof(G->cell)) != 0)
+      return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: grid %d cell differs from the framework unit cell", g);
+    if (types[g] != RG_GRID_LENNARD_JONES && types[g] != RG_GRID_EWALD_REAL)
+      return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_build: grid type %d not supported", int(types[g]));
+    if (types[g] == RG_GRID_LENNARD_JONES && (probes[g] < 0 || probes[g] >= ff->n))
+      return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: probe pseudo-atom %d out of range", probes[g]);
+    for (int a = 0; a < 3; ++a)
+    {
+      const double ncell = std::ceil(2.0 * cutoffs[g] / fw->widths[a]);
+      if (ncell * fw->widths[a] < 2.0 * ff->rc_framework)
+        return rg::fail(ctx, RG_ERR_UNSUPPORTED,
+                        "rg_grid_build: supercell from cutoff %g spans less than twice the VDW cutoff %g",
+                        cutoffs[g], ff->rc_framework);
+    }
+  }
+  if (g0->n[0] < 2 || g0->n[1] < 2 || g0->n[2] < 2)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: at least 2 points per axis");
+  for (int t : fw->type)
+    if (t < 0 || t >= ff->n) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: framework type out of range");
+
+  // local rows for the framework's distinct pseudo-atom types
+  std::vector<int> typeRow(ff->n, 0), rowType;
+  {
+    std::vector<int> seen(ff->n, -1);
+    for (int t : fw->type)
+      if (seen[t] < 0)
+      {
+        seen[t] = static_cast<int>(rowType.size());
+        rowType.push_back(t);
+      }
+    if (rowType.size() > size_t(MAXT))
+      return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_build: more than %d framework pseudo-atom types", MAXT);
+    for (int t = 0; t < ff->n; ++t) typeRow[t] = seen[t] < 0 ? 0 : seen[t];
+  }
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  if (fw->images.rc != ff->rc_framework)
+  {
+    rg_status s = buildImages(fw, ff->rc_framework, typeRow);
+    if (s != RG_OK) return s;
+  }
+
+  // split into launches of <= MAXG LJ grids; the Coulomb grid (at most one) rides with the first
+  std::vector<int> lj, coul;
+  for (int g = 0; g < n_grids; ++g) (types[g] == RG_GRID_LENNARD_JONES ? lj : coul).push_back(g);
+  if (coul.size() > 1)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: at most one Ewald real-space grid per call");
+
+  GridArgs A{};
+  std::memcpy(A.cell, fw->cell, sizeof(A.cell));
+  const int* n = g0->n;
+  for (int a = 0; a < 3; ++a)
+  {
+    A.n[a] = n[a];
+    A.delta[a] = 1.0 / double(n[a] - 1);
+    A.ncm1[a] = double(n[a] - 1);
+    A.diag[a] = fw->cell[4 * a];
+    A.lo[a] = fw->images.lo[a];
+    A.bw[a] = fw->images.bw[a];
+    A.nb[a] = fw->images.nb[a];
+  }
+  const double* H = fw->cell;
+  A.G01 = H[0] * H[3] + H[1] * H[4] + H[2] * H[5];
+  A.G02 = H[0] * H[6] + H[1] * H[7] + H[2] * H[8];
+  A.G12 = H[3] * H[6] + H[4] * H[7] + H[5] * H[8];
+  const double rc = ff->rc_framework;
+  A.rc2 = rc * rc;
+  // tile half-diagonal
+  double rt = 0.0;
+  {
+    const int T[3] = {TI, TJ, TK};
+    double d[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) d[a][c] = 0.5 * (T[a] - 1) * A.delta[a] * H[3 * a + c];
+    for (int sa = -1; sa <= 1; sa += 2)
+      for (int sb = -1; sb <= 1; sb += 2)
+      {
+        double v[3];
+        for (int c = 0; c < 3; ++c) v[c] = d[0][c] + sa * d[1][c] + sb * d[2][c];
+        rt = std::max(rt, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+  }
+  const double R = rc + rt + 1e-6;
+  A.cut2 = R * R;
+  for (int a = 0; a < 3; ++a) A.ext[a] = R / fw->widths[a] + 1e-9;
+  for (int a = 0; a < 3; ++a)
+  {
+    const int span = static_cast<int>(std::ceil(2.0 * A.ext[a] / A.bw[a])) + 2;
+    if (a > 0 && span * span > MAXROWS)
+      return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_build: bin box too large for the row table");
+  }
+  A.ntile[0] = (n[0] + TI - 1) / TI;
+  A.ntile[1] = (n[1] + TJ - 1) / TJ;
+  A.ntile[2] = (n[2] + TK - 1) / TK;
+  A.ntiles = A.ntile[0] * A.ntile[1] * A.ntile[2];
+  A.img = static_cast<const Img*>(fw->images.d_img);
+  A.bin_start = fw->images.d_bin_start;
+  A.alpha = ff->alpha;
+  A.alpha2 = ff->alpha * ff->alpha;
+  A.k2a = 2.0 * ff->alpha / std::sqrt(M_PI);
+  A.coulomb = rg::units().coulomb;
+  A.overlap = ff->overlap;
+
+  const bool tric = !fw->rect;
+  // parameter tables of all launches, copied once into the context's persistent table buffer
+  struct Launch
+  {
+    int nlj;
+    bool coul;
+    double* out[MAXG + 1];
+  };
+  std::vector<Launch> launches;
+  std::vector<LJPar> table;
+  {
+    size_t li = 0;
+    bool coulDone = coul.empty();
+    while (li < lj.size() || !coulDone)
+    {
+      Launch L{};
+      L.nlj = static_cast<int>(std::min<size_t>(MAXG, lj.size() - li));
+      L.coul = !coulDone;
+      const size_t t0 = table.size();
+      table.resize(t0 + MAXG * MAXT, LJPar{});
+      for (int q = 0; q < L.nlj; ++q)
+      {
+        const int g = lj[li + q];
+        const int ta = probes[g];
+        for (size_t row = 0; row < rowType.size(); ++row)
+        {
+          const size_t k = size_t(ta) * ff->n + rowType[row];
+          LJPar P{};
+          P.e4 = 4.0 * ff->eps[k];
+          P.s2 = ff->sigma[k] * ff->sigma[k];
+          P.sh = ff->shift[k];
+          P.c1 = -6.0 * P.e4;
+          P.c2 = 24.0 * P.e4;
+          P.c3 = -96.0 * P.e4;
+          table[t0 + q * MAXT + row] = P;
+        }
+        L.out[q] = grids[g]->d;
+      }
+      if (L.coul) L.out[L.nlj] = grids[coul[0]]->d;
+      launches.push_back(L);
+      li += L.nlj;
+      coulDone = true;
+    }
+  }
+  const size_t bytes = table.size() * sizeof(LJPar);
+  if (ctx->table_bytes < bytes)
+  {
+    // waits for earlier launches that still read the old table before it is released
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->d_table) hipFree(ctx->d_table);
+    ctx->d_table = nullptr;
+    ctx->table_bytes = 0;
+    RG_HIP(ctx, hipMalloc(&ctx->d_table, bytes));
+    ctx->table_bytes = bytes;
+  }
+  RG_HIP(ctx, hipMemcpyAsync(ctx->d_table, table.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+  for (size_t l = 0; l < launches.size(); ++l)
+  {
+    const Launch& L = launches[l];
+    for (int q = 0; q <= MAXG; ++q) A.out[q] = L.out[q];
+    A.ljpar = static_cast<const LJPar*>(ctx->d_table) + l * MAXG * MAXT;
+    KernelFn fn = pick(L.nlj, L.coul, tric);
+    if (!fn) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: nothing to build");
+    hipLaunchKernelGGL(fn, dim3(A.ntiles), dim3(256), 0, ctx->stream, A);
+    RG_HIP(ctx, hipGetLastError());
+  }
+  if (sync) RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+}  // namespace
+
+extern "C"
+{
+
+rg_status rg_grid_create(rg_ctx* ctx, const double* cell9, int is_rectangular, int nx, int ny, int nz, rg_grid** out)
+{
+  if (!ctx || !cell9 || !out || nx < 2 || ny < 2 || nz < 2)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_create: invalid arguments");
+  rg_grid* g = new rg_grid();
+  g->ctx = ctx;
+  g->device = ctx->device;
+  std::memcpy(g->cell, cell9, sizeof(g->cell));
+  rg::invert3(g->cell, g->inv);
+  g->rect = is_rectangular;
+  g->n[0] = nx;
+  g->n[1] = ny;
+  g->n[2] = nz;
+  g->count = 8 * size_t(nx) * size_t(ny) * size_t(nz);
+  hipSetDevice(ctx->device);
+  hipError_t e = hipMalloc(&g->d, g->count * sizeof(double));
+  if (e != hipSuccess)
+  {
+    delete g;
+    return rg::hipfail(ctx, e, "hipMalloc(grid)");
+  }
+  *out = g;
+  return RG_OK;
+}
+
+rg_status rg_grid_destroy(rg_grid* g)
+{
+  if (!g) return RG_ERR_INVALID;
+  hipSetDevice(g->device);
+  if (g->d) hipFree(g->d);
+  delete g;
+  return RG_OK;
+}
+
+void* rg_grid_device_data(rg_grid* g) { return g ? g->d : nullptr; }
+
+rg_status rg_grid_build(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, rg_grid_type type,
+                        int probe_pseudo_atom, double cutoff_supercell, rg_grid* grid)
+{
+  rg_grid* gs[1] = {grid};
+  return gridBuildMany(ctx, ff, fw, 1, &type, &probe_pseudo_atom, &cutoff_supercell, gs, true);
+}
+
+rg_status rg_grid_build_many(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, int n_grids,
+                             const rg_grid_type* types, const int* probes, const double* cutoffs,
+                             rg_grid* const* grids)
+{
+  return gridBuildMany(ctx, ff, fw, n_grids, types, probes, cutoffs, grids, true);
+}
+
+rg_status rg_grid_build_many_async(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, int n_grids,
+                                   const rg_grid_type* types, const int* probes, const double* cutoffs,
+                                   rg_grid* const* grids)
+{
+  return gridBuildMany(ctx, ff, fw, n_grids, types, probes, cutoffs, grids, false);
+}
+
+rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size_t count, double* host)
+{
+  if (!ctx || !grid || !host || offset + count > grid->count)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_download: range out of bounds");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_HIP(ctx, hipMemcpyAsync(host, grid->d + offset, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count, const double* host)
+{
+  if (!ctx || !grid || !host || offset + count > grid->count)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_upload: range out of bounds");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_HIP(ctx, hipMemcpyAsync(grid->d + offset, host, count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""Synthetic, seeded workloads: frameworks, force fields and adsorbates of the five benchmark configurations.
+
+Every choice is recorded here and seeded, so that the same inputs are produced on any machine.  The generated
+frameworks are cached as ``.npz`` files under ``raspa3_amd/data`` (committed), and regenerated from the seeds if
+the cache is absent.
+
+Construction (configurations 0 and 1, and the per-system frameworks of 2 and 4):
+  * Si on a jittered lattice: site (i+1/2, j+1/2, k+1/2)/n in fractional coordinates, mapped to Cartesian, plus
+    a uniform jitter of +-0.25 A per Cartesian axis (numpy PCG64, seed ``seed_si``).
+  * Two O per Si at 1.6 A along random directions (seed ``seed_o``, batches of 256 directions); a candidate is
+    rejected if it comes closer than ``d_oo`` to an O already placed or closer than 1.2 A to any other Si
+    (minimum image), and of the admissible directions of a batch the one farthest from the other Si is kept.  If some Si cannot
+    place its O within the attempt budget, the whole O placement restarts with the next seed; the seed actually
+    used is stored with the framework.
+  * Configuration 0: orthorhombic 20.0 x 19.9 x 13.4 A, 4 x 4 x 6 Si (96 Si + 192 O), d_oo = 2.4 A.
+  * Configuration 1: a = b = 40, c = 30 A, gamma = 100 deg; 13 x 15 x 7 Si = 1365 Si + 2730 O = 4095 sites, exactly
+    charge neutral with Si +2.05 e and O -1.025 e (4096 sites cannot be split 1:2).  d_oo = 2.0 A: 2730 O in
+    47,271 A^3 (0.0578 A^-3) lies above the random-sequential-addition jamming density of 2.4 A spheres
+    (about 0.053 A^-3), so 2.4 A cannot be reached by this construction.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def cell_from_lengths_angles(a, b, c, alpha=90.0, beta=90.0, gamma=90.0):
+    """Cell matrix with lattice vectors as columns, as SimulationBox(a, b, c, alpha, beta, gamma) builds it
+    (src/raspakit/simulationbox.cpp:40-60).  Returns (cell 3x3, is_rectangular)."""
+    al, be, ga = (math.pi * x / 180.0 for x in (alpha, beta, gamma))
+    rect = abs(al - 0.5 * math.pi) < 1e-5 and abs(be - 0.5 * math.pi) < 1e-5 and abs(ga - 0.5 * math.pi) < 1e-5
+    t = (math.cos(al) - math.cos(ga) * math.cos(be)) / math.sin(ga)
+    H = np.zeros((3, 3))
+    H[:, 0] = (a, 0.0, 0.0)
+    H[:, 1] = (b * math.cos(ga), b * math.sin(ga), 0.0)
+    H[:, 2] = (c * math.cos(be), c * t, c * math.sqrt(1.0 - math.cos(be) ** 2 - t * t))
+    return H, rect
+
+
+def cell_orthorhombic(a, b, c):
+    """SimulationBox(a, b, c): exactly diagonal (src/raspakit/simulationbox.cpp:15-37)."""
+    return np.diag([a, b, c]).astype(float), True
+
+
+@dataclass
+class Framework:
+    name: str
+    cell: np.ndarray  # 3x3, columns = lattice vectors
+    rect: bool
+    pos: np.ndarray  # (n, 3)
+    type: np.ndarray  # (n,) pseudo-atom index
+    charge: np.ndarray  # (n,)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.type)
+
+    def cell9(self):
+        return np.ascontiguousarray(self.cell.T.reshape(9))  # columns consecutive
+
+
+def _min_image_dist(p, X, H, Hinv):
+    if len(X) == 0:
+        return np.full(p.shape[0], np.inf)
+    d = X[None, :, :] - p[:, None, :]
+    s = d @ Hinv.T
+    s -= np.rint(s)
+    d = s @ H.T
+    return np.sqrt((d * d).sum(-1)).min(1)
+
+
+def generate_silicate(name, H, rect, nsi, d_oo, seed_si, seed_o, type_si, type_o, q_si, q_o, jitter=0.25,
+                      d_sio=1.6, d_osi=1.2, batch=256, max_batches=16, max_restarts=400):
+    """Jittered Si lattice + two O per Si; see the module docstring."""
+    Hinv = np.linalg.inv(H)
+    rng = np.random.default_rng(seed_si)
+    si = []
+    for k in range(nsi[2]):
+        for j in range(nsi[1]):
+            for i in range(nsi[0]):
+                f = np.array([(i + 0.5) / nsi[0], (j + 0.5) / nsi[1], (k + 0.5) / nsi[2]])
+                si.append(H @ f + rng.uniform(-jitter, jitter, 3))
+    si = np.array(si)
+    nsi_tot = len(si)
+    for restart in range(max_restarts):
+        seed = seed_o + restart
+        rng = np.random.default_rng(seed)
+        O = np.zeros((0, 3))
+        ok_all = True
+        for n in range(nsi_tot):
+            others = np.delete(si, n, 0)
+            for _ in range(2):
+                placed = False
+                for _b in range(max_batches):
+                    v = rng.normal(size=(batch, 3))
+                    v /= np.linalg.norm(v, axis=1, keepdims=True)
+                    p = si[n] + d_sio * v
+                    dSi = _min_image_dist(p, others, H, Hinv)
+                    good = (_min_image_dist(p, O, H, Hinv) >= d_oo) & (dSi >= d_osi)
+                    idx = np.flatnonzero(good)
+                    if len(idx):
+                        # of the admissible directions, keep the one farthest from the other Si (the O stays
+                        # inside its own Si's Voronoi cell and leaves room for the neighbours' O)
+                        O = np.vstack([O, p[idx[np.argmax(dSi[idx])]]])
+                        placed = True
+                        break
+                if not placed:
+                    ok_all = False
+                    break
+            if not ok_all:
+                break
+        if ok_all:
+            pos = np.vstack([si, O])
+            typ = np.array([type_si] * nsi_tot + [type_o] * len(O), dtype=np.int32)
+            chg = np.array([q_si] * nsi_tot + [q_o] * len(O), dtype=float)
+            meta = dict(seed_si=seed_si, seed_o_used=seed, d_oo=d_oo, nsi=list(nsi), jitter=jitter)
+            return Framework(name, H, rect, pos, typ, chg, meta)
+    raise RuntimeError(f"{name}: O placement failed after {max_restarts} restarts")
+
+
+def _cached(name, make):
+    path = os.path.join(DATA_DIR, name + ".npz")
+    if os.path.exists(path):
+        z = np.load(path, allow_pickle=False)
+        return Framework(name, z["cell"], bool(z["rect"]), z["pos"], z["type"], z["charge"],
+                         json.loads(str(z["meta"])))
+    fw = make()
+    os.makedirs(DATA_DIR, exist_ok=True)
+    np.savez(path, cell=fw.cell, rect=fw.rect, pos=fw.pos, type=fw.type, charge=fw.charge,
+             meta=json.dumps(fw.meta))
+    return fw
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# Force fields.  Pseudo-atom tables as (name, eps_K, sigma_A, charge, is_framework).
+# ---------------------------------------------------------------------------------------------------------------
+
+@dataclass
+class ForceFieldSpec:
+    names: list
+    eps_K: np.ndarray
+    sigma: np.ndarray
+    charge: np.ndarray
+    is_framework: np.ndarray
+    shifted: bool = True
+    rc_framework: float = 12.0
+    rc_molecule: float = 12.0
+    rc_coulomb: float = 12.0
+    alpha: float = 0.265
+    use_charge: bool = True
+    overlap: float = 1e6
+
+    def index(self, name):
+        return self.names.index(name)
+
+
+def _ff(rows, **kw):
+    names = [r[0] for r in rows]
+    return ForceFieldSpec(names, np.array([r[1] for r in rows], float), np.array([r[2] for r in rows], float),
+                          np.array([r[3] for r in rows], float), np.array([int(r[4]) for r in rows], np.int32), **kw)
+
+
+def forcefield_config0():
+    """LJ on O only (44 K, 3.3 A), Si without LJ; united-atom CH4 (148 K, 3.73 A); LB mixing, 12 A, shifted."""
+    return _ff([("Si", 0.0, 1.0, 0.0, True), ("O", 44.0, 3.3, 0.0, True), ("CH4", 148.0, 3.73, 0.0, False)],
+               use_charge=False)
+
+
+def forcefield_config1():
+    """Charged silicate (Si +2.05, O -1.025) with CO2 (C 27 K 2.80 A +0.70; O 79 K 3.05 A -0.35); alpha 0.265."""
+    return _ff([("Si", 0.0, 1.0, 2.05, True), ("O", 44.0, 3.3, -1.025, True),
+                ("C_co2", 27.0, 2.80, 0.70, False), ("O_co2", 79.0, 3.05, -0.35, False)],
+               alpha=0.265, use_charge=True)
+
+
+def framework_config0():
+    H, rect = cell_orthorhombic(20.0, 19.9, 13.4)
+    return _cached("config0_framework", lambda: generate_silicate(
+        "config0_framework", H, rect, (6, 4, 4), 2.4, 1000, 1001, 0, 1, 0.0, 0.0))
+
+
+def framework_config1():
+    H, rect = cell_from_lengths_angles(40.0, 40.0, 30.0, 90.0, 90.0, 100.0)
+    return _cached("config1_framework", lambda: generate_silicate(
+        "config1_framework", H, rect, (13, 15, 7), 2.0, 1000, 1001, 0, 1, 2.05, -1.025))
+
+
+def grid_dims(cell, spacing):
+    """n = (size_t)(w / h + 0.5) from perpendicular widths (src/raspakit/system_interpolation_grids.cpp:222-226)."""
+    a, b, c = cell[:, 0], cell[:, 1], cell[:, 2]
+    c1, c2, c3 = np.cross(a, b), np.cross(b, c), np.cross(a, c)
+    v = abs(a @ c2)
+    w = np.array([v / math.sqrt(c2 @ c2), v / math.sqrt(c3 @ c3), v / math.sqrt(c1 @ c1)])
+    return [int(x / spacing + 0.5) for x in w], w

@@ -1,0 +1,154 @@
+"""GPU parity of the grid builder and tricubic interpolation against the FP64 oracle (through the C ABI).
+
+Tolerance (per stored value, internal energy units): |gpu - oracle| <= 1e-3 K + 1e-9 |oracle|.  The absolute
+term is the bar the GPU path is held to on energies; the relative term covers the derivative channels, whose
+values near framework atoms reach 1e8 and more, where FP64 summation order alone moves the last digits.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import raspa3_amd as ra
+from raspa3_amd import workloads as w
+from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES, RGError
+
+pytestmark = pytest.mark.gpu
+
+K_TO_E = oracle.units()["kelvin_to_energy"]
+ABS = 1e-3 * K_TO_E
+REL = 1e-9
+NTHREADS = 16
+
+
+def assert_close(gpu, ref, what):
+    err = np.abs(gpu - ref)
+    bound = ABS + REL * np.abs(ref)
+    bad = np.flatnonzero(err > bound)
+    assert bad.size == 0, f"{what}: {bad.size} values out of tolerance; worst {err.reshape(-1)[bad[:5]]} " \
+                          f"ref {np.asarray(ref).reshape(-1)[bad[:5]]}"
+
+
+def test_config0_full_grid_matches_oracle(gpu_ctx):
+    """CH4 LJ grid, orthorhombic 20.0 x 19.9 x 13.4, h = 0.2 -> 100 x 99 x 67, 2x2x2 supercell: every point."""
+    print("case: config0 full LJ grid", flush=True)
+    fw, spec = w.framework_config0(), w.forcefield_config0()
+    dims = ra.grid_dims(fw.cell, 0.2)
+    assert dims == [100, 99, 67]
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    g = ra.Grid(gpu_ctx, fw.cell, fw.rect, dims)
+    ra.build_grids(gpu_ctx, ff, f, [(g, RG_GRID_LENNARD_JONES, spec.index("CH4"), spec.rc_framework)])
+    got = g.download().reshape(dims[2], dims[1], dims[0], 8)
+    ref = oracle.grid_build(fw.cell, fw.rect, dims, 0, fw.pos, fw.type, fw.charge, oracle.pairs_from_spec(spec),
+                            spec.index("CH4"), 12.0, 12.0, spec.alpha, nthreads=NTHREADS)
+    assert_close(got, ref, "config0")
+    # overlap clamps are present and reproduced exactly
+    clamped = ref[..., 0] == 2.0 * spec.overlap
+    assert clamped.any()
+    assert np.array_equal(got[..., 0][clamped], ref[..., 0][clamped])
+
+
+def config1_grids(ctx):
+    fw, spec = w.framework_config1(), w.forcefield_config1()
+    dims = ra.grid_dims(fw.cell, 0.1)
+    ff = ra.ForceField(ctx, spec)
+    f = ra.Framework(ctx, fw)
+    grids = [ra.Grid(ctx, fw.cell, fw.rect, dims) for _ in range(3)]
+    specs = [(grids[0], RG_GRID_LENNARD_JONES, spec.index("C_co2"), 12.0),
+             (grids[1], RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0),
+             (grids[2], RG_GRID_EWALD_REAL, 0, 12.0)]
+    return fw, spec, dims, ff, f, grids, specs
+
+
+def sample_points(dims, n, seed):
+    rng = np.random.default_rng(seed)
+    ijk = np.stack([rng.integers(0, d, n) for d in dims], 1)
+    corners = np.array([[i, j, k] for i in (0, dims[0] - 1) for j in (0, dims[1] - 1) for k in (0, dims[2] - 1)])
+    return np.vstack([ijk, corners]).astype(np.int32)
+
+
+def gather(grid, dims, ijk):
+    out = np.empty((len(ijk), 8))
+    for q, (i, j, k) in enumerate(ijk):
+        off = 8 * (int(i) + dims[0] * (int(j) + dims[1] * int(k)))
+        out[q] = grid.download(off, 8)
+    return out
+
+
+def test_config1_fused_grids_match_oracle(gpu_ctx):
+    """Triclinic 40 x 40 x 30, gamma 100, 4095 charged sites, h = 0.1 -> 394 x 394 x 300 (46.6M points per grid):
+    C_co2 and O_co2 LJ grids and the unit-charge Ewald real-space grid, built in one fused pass at full size;
+    2,000 sampled points plus the 8 corners are compared with the oracle."""
+    print("case: config1 fused grids", flush=True)
+    fw, spec, dims, ff, f, grids, specs = config1_grids(gpu_ctx)
+    ra.build_grids(gpu_ctx, ff, f, specs)
+    ijk = sample_points(dims, 2000, 5)
+    pairs = oracle.pairs_from_spec(spec)
+    for (g, gt, probe, _), name in zip(specs, ["C_co2", "O_co2", "ewald"]):
+        ref = oracle.grid_points(fw.cell, fw.rect, dims, gt, fw.pos, fw.type, fw.charge, pairs, probe, 12.0, 12.0,
+                                 spec.alpha, ijk, nthreads=NTHREADS)
+        assert_close(gather(g, dims, ijk), ref, f"config1 {name}")
+
+
+def test_fused_equals_single_and_is_deterministic(gpu_ctx):
+    """Size-independent properties at full config-1 size: a grid built alone equals the same grid from the fused
+    pass bit for bit, and rebuilding gives identical bits."""
+    print("case: fused vs single", flush=True)
+    fw, spec, dims, ff, f, grids, specs = config1_grids(gpu_ctx)
+    ra.build_grids(gpu_ctx, ff, f, specs)
+    fused = [g.download(0, 8 * dims[0] * dims[1] * 20) for g in grids]
+    single = ra.Grid(gpu_ctx, fw.cell, fw.rect, dims)
+    for q, (g, gt, probe, cut) in enumerate(specs):
+        ra.build_grids(gpu_ctx, ff, f, [(single, gt, probe, cut)])
+        assert np.array_equal(single.download(0, fused[q].size), fused[q])
+    ra.build_grids(gpu_ctx, ff, f, specs)
+    assert np.array_equal(grids[0].download(0, fused[0].size), fused[0])
+
+
+def test_small_triclinic_with_supercell(gpu_ctx):
+    """A small triclinic cell (supercell 2x2x2 for rc = 12) with a 2-point axis, all grid types, every point."""
+    print("case: small triclinic", flush=True)
+    cell, rect = w.cell_from_lengths_angles(9.0, 10.0, 11.0, 80.0, 95.0, 105.0)
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0, 1, (10, 3)) @ cell.T
+    typ = np.array([0, 1] * 5, np.int32)
+    q = np.array([1.3, -1.3] * 5)
+    fw = w.Framework("small", cell, rect, pos, typ, q)
+    spec = w.forcefield_config1()
+    dims = [5, 2, 7]
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    gs = [ra.Grid(gpu_ctx, cell, rect, dims) for _ in range(2)]
+    ra.build_grids(gpu_ctx, ff, f, [(gs[0], RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0),
+                                    (gs[1], RG_GRID_EWALD_REAL, 0, 12.0)])
+    pairs = oracle.pairs_from_spec(spec)
+    for g, gt, pr in ((gs[0], 0, spec.index("O_co2")), (gs[1], 3, 0)):
+        ref = oracle.grid_build(cell, rect, dims, gt, pos, typ, q, pairs, pr, 12.0, 12.0, spec.alpha)
+        assert_close(g.download().reshape(ref.shape), ref, f"small type {gt}")
+
+
+def test_supercell_narrower_than_cutoff_is_rejected(gpu_ctx):
+    fw, spec = w.framework_config0(), w.forcefield_config0()
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    g = ra.Grid(gpu_ctx, fw.cell, fw.rect, [10, 10, 10])
+    with pytest.raises(RGError, match="supercell"):
+        ra.build_grids(gpu_ctx, ff, f, [(g, RG_GRID_LENNARD_JONES, spec.index("CH4"), 5.0)])
+
+
+def test_interpolation_matches_oracle(gpu_ctx):
+    """Tricubic interpolation of the config-0 grid at 20,000 random positions (inside and outside the cell)."""
+    print("case: interpolation", flush=True)
+    fw, spec = w.framework_config0(), w.forcefield_config0()
+    dims = ra.grid_dims(fw.cell, 0.2)
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    g = ra.Grid(gpu_ctx, fw.cell, fw.rect, dims)
+    ra.build_grids(gpu_ctx, ff, f, [(g, RG_GRID_LENNARD_JONES, spec.index("CH4"), 12.0)])
+    data = g.download()
+    rng = np.random.default_rng(9)
+    pos = rng.uniform(-0.5, 1.5, (20000, 3)) @ fw.cell.T
+    e, grad = g.interpolate(pos, gradient=True)
+    eo, go = oracle.interpolate(fw.cell, fw.rect, dims, data, pos, gradient=True)
+    assert_close(e, eo, "interpolate")
+    assert np.allclose(grad, go, rtol=1e-9, atol=1e-6)
