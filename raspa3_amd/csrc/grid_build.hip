@@ -172,16 +172,18 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     pu2 = A.cell[6] * px + A.cell[7] * py + A.cell[8] * pz;
   }
 
-  // tile centre (fractional and Cartesian) and the bin box that holds every image within rc of the tile
-  const double fc0 = (double(tx * TI) + 0.5 * (TI - 1)) * A.delta[0];
-  const double fc1 = (double(ty * TJ) + 0.5 * (TJ - 1)) * A.delta[1];
-  const double fc2 = (double(tz * TK) + 0.5 * (TK - 1)) * A.delta[2];
+  // centre of the tile's points (fractional and Cartesian; a tile at the grid edge holds fewer points) and the
+  // bin box that holds every image within rc of them
+  const double fc0 = 0.5 * double(2 * tx * TI + min(TI, A.n[0] - tx * TI) - 1) * A.delta[0];
+  const double fc1 = 0.5 * double(2 * ty * TJ + min(TJ, A.n[1] - ty * TJ) - 1) * A.delta[1];
+  const double fc2 = 0.5 * double(2 * tz * TK + min(TK, A.n[2] - tz * TK) - 1) * A.delta[2];
   const double cxc = A.cell[0] * fc0 + A.cell[3] * fc1 + A.cell[6] * fc2;
   const double cyc = A.cell[1] * fc0 + A.cell[4] * fc1 + A.cell[7] * fc2;
   const double czc = A.cell[2] * fc0 + A.cell[5] * fc1 + A.cell[8] * fc2;
   int blo[3], bhi[3];
   {
     const double fc[3] = {fc0, fc1, fc2};
+#pragma unroll
     for (int a = 0; a < 3; ++a)
     {
       int l = int(floor((fc[a] - A.ext[a] - A.lo[a]) / A.bw[a]));
@@ -236,7 +238,7 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     // ---- gather + sphere filter + compaction into LDS
     const int g = base + tid;
     bool pass = false;
-    Img im;
+    double2 r0, r1, r2, r3;  // the image record as (x, y), (z, q), (u0, u1), (u2, type)
     if (g < total)
     {
       int lo = 0, hi = nrow - 1;
@@ -248,12 +250,17 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
         else
           hi = mid - 1;
       }
-      im = A.img[rowStart[lo] + (g - rowPrefix[lo])];
-      const double dx = im.x - cxc, dy = im.y - cyc, dz = im.z - czc;
+      const double2* src = reinterpret_cast<const double2*>(A.img + rowStart[lo] + (g - rowPrefix[lo]));
+      r0 = src[0];
+      r1 = src[1];
+      r2 = src[2];
+      r3 = src[3];
+      const int type = static_cast<int>(__double_as_longlong(r3.y) & 0xffffffffll);
+      const double dx = r0.x - cxc, dy = r0.y - cyc, dz = r1.x - czc;
       pass = dx * dx + dy * dy + dz * dz < A.cut2;
-      bool useful = COUL && im.q != 0.0;
+      bool useful = COUL && r1.y != 0.0;
 #pragma unroll
-      for (int q = 0; q < NLJ; ++q) useful = useful || par[q][im.type].e4 != 0.0;
+      for (int q = 0; q < NLJ; ++q) useful = useful || par[q][type].e4 != 0.0;
       pass = pass && useful;
     }
     const unsigned long long m = __ballot(pass);
@@ -264,8 +271,12 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     const int ncand = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
     if (pass)
     {
-      im.q *= A.coulomb;
-      cand[off + __popcll(m & lt_mask)] = im;
+      r1.y *= A.coulomb;
+      double2* dst = reinterpret_cast<double2*>(&cand[off + __popcll(m & lt_mask)]);
+      dst[0] = r0;
+      dst[1] = r1;
+      dst[2] = r2;
+      dst[3] = r3;
     }
     __syncthreads();
 
@@ -414,11 +425,12 @@ KernelFn pick(int nlj, bool coul, bool tric)
 
 // Builds (once per framework and cutoff) the binned periodic images of the unit-cell atoms that can lie within
 // rc of a point of the closed unit cell [0,1]^3.
-rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeRow)
+rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeRow, double binTarget)
 {
   ImageSet& S = fw->images;
-  if (S.rc == rc && S.d_img) return RG_OK;
   rg_ctx* ctx = fw->ctx;
+  // earlier builds may still read the old images
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (S.d_img) hipFree(S.d_img);
   if (S.d_bin_start) hipFree(S.d_bin_start);
   S.d_img = nullptr;
@@ -431,7 +443,7 @@ rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeR
     m[a] = rc / fw->widths[a] + 1e-9;
     S.lo[a] = -m[a];
     const double span = 1.0 + 2.0 * m[a];
-    S.nb[a] = std::max(1, std::min(512, static_cast<int>(span * fw->widths[a] / 2.0)));
+    S.nb[a] = std::max(1, std::min(512, static_cast<int>(span * fw->widths[a] / binTarget)));
     S.bw[a] = span / S.nb[a];
   }
   std::vector<Img> imgs;
@@ -486,6 +498,7 @@ rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeR
   RG_HIP(ctx, hipMemcpyAsync(S.d_bin_start, start.data(), (nbins + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   S.rc = rc;
+  S.binTarget = binTarget;
   return RG_OK;
 }
 
@@ -538,9 +551,30 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
     for (int t = 0; t < ff->n; ++t) typeRow[t] = seen[t] < 0 ? 0 : seen[t];
   }
   RG_HIP(ctx, hipSetDevice(ctx->device));
-  if (fw->images.rc != ff->rc_framework)
+  const int* n = g0->n;
+  const double* H = fw->cell;
+  const double rc = ff->rc_framework;
+  // half-diagonal of the largest tile of points (a tile never spans more points than the grid has)
+  double rt = 0.0;
   {
-    rg_status s = buildImages(fw, ff->rc_framework, typeRow);
+    const int T[3] = {std::min(TI, n[0]), std::min(TJ, n[1]), std::min(TK, n[2])};
+    double d[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) d[a][c] = 0.5 * (T[a] - 1) / double(n[a] - 1) * H[3 * a + c];
+    for (int sa = -1; sa <= 1; sa += 2)
+      for (int sb = -1; sb <= 1; sb += 2)
+      {
+        double v[3];
+        for (int c = 0; c < 3; ++c) v[c] = d[0][c] + sa * d[1][c] + sb * d[2][c];
+        rt = std::max(rt, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+  }
+  const double R = rc + rt + 1e-6;
+  // bins of at least 2 A, and wide enough that the bin box of a tile spans at most 30 x 30 rows
+  const double binTarget = std::max(2.0, 2.0 * R / 28.0);
+  if (fw->images.rc != rc || fw->images.binTarget < binTarget)
+  {
+    rg_status s = buildImages(fw, rc, typeRow, binTarget);
     if (s != RG_OK) return s;
   }
 
@@ -552,7 +586,6 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
 
   GridArgs A{};
   std::memcpy(A.cell, fw->cell, sizeof(A.cell));
-  const int* n = g0->n;
   for (int a = 0; a < 3; ++a)
   {
     A.n[a] = n[a];
@@ -563,28 +596,10 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
     A.bw[a] = fw->images.bw[a];
     A.nb[a] = fw->images.nb[a];
   }
-  const double* H = fw->cell;
   A.G01 = H[0] * H[3] + H[1] * H[4] + H[2] * H[5];
   A.G02 = H[0] * H[6] + H[1] * H[7] + H[2] * H[8];
   A.G12 = H[3] * H[6] + H[4] * H[7] + H[5] * H[8];
-  const double rc = ff->rc_framework;
   A.rc2 = rc * rc;
-  // tile half-diagonal
-  double rt = 0.0;
-  {
-    const int T[3] = {TI, TJ, TK};
-    double d[3][3];
-    for (int a = 0; a < 3; ++a)
-      for (int c = 0; c < 3; ++c) d[a][c] = 0.5 * (T[a] - 1) * A.delta[a] * H[3 * a + c];
-    for (int sa = -1; sa <= 1; sa += 2)
-      for (int sb = -1; sb <= 1; sb += 2)
-      {
-        double v[3];
-        for (int c = 0; c < 3; ++c) v[c] = d[0][c] + sa * d[1][c] + sb * d[2][c];
-        rt = std::max(rt, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-      }
-  }
-  const double R = rc + rt + 1e-6;
   A.cut2 = R * R;
   for (int a = 0; a < 3; ++a) A.ext[a] = R / fw->widths[a] + 1e-9;
   for (int a = 0; a < 3; ++a)

@@ -37,6 +37,7 @@ struct rg_forcefield
 struct ImageSet
 {
   double rc = -1.0;
+  double binTarget = 0.0;  // minimum bin width (A) the bins were built with
   int n_images = 0;
   int nb[3] = {0, 0, 0};
   double lo[3] = {0, 0, 0}, bw[3] = {0, 0, 0};
