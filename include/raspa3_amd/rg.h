@@ -63,6 +63,10 @@ const char* rg_version(void);
 rg_status rg_box(double a, double b, double c, double alpha_deg, double beta_deg, double gamma_deg, int lengths_only,
                  double* cell9, double* inv9, int* is_rectangular, double* volume, double* widths3);
 
+/* ForceField::initializeEwaldParameters with automaticEwald (src/raspakit/forcefield/forcefield.cpp:1415-1438):
+ * alpha and kmax3 from EwaldPrecision, the Coulomb cutoff and the perpendicular widths. */
+rg_status rg_ewald_parameters(double precision, double rc_coulomb, const double* widths3, double* alpha, int* kmax3);
+
 /* Grid dimensions from a spacing: n = (size_t)(w/h + 0.5) per axis,
  * src/raspakit/system_interpolation_grids.cpp:192-198,222-226. */
 rg_status rg_grid_dims(const double* cell9, double spacing, int* n3);
@@ -136,6 +140,127 @@ rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count
  * d/dr (interpolateGradient, :4189-4347) when non-null. */
 rg_status rg_grid_interpolate(rg_ctx* ctx, const rg_grid* grid, int n, const double* pos, double* energy,
                               double* gradient);
+
+/* ---------------------------------------------------------------------------------------------
+ * Systems: what CBMC::GrowContext (src/raspakit/cbmc/cbmc_growth_context.ixx:15-29) gives the trial-energy code —
+ * box, force field, framework atoms or interpolation grids, adsorbate atoms — plus the Ewald state
+ * (System::storedEik, src/raspakit/system.ixx:235-241), all device-resident.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rg_system rg_system;
+
+/* fw: framework atoms for the direct minimum-image sum (may be NULL).  grids[g] is the LJ grid of pseudo-atom
+ * grid_types[g] (System::interpolationGrids[type], src/raspakit/system.ixx:296); coulomb_grid the unit-charge
+ * grid (interpolationGrids.back()) or NULL.  A site whose type has a grid uses it, as in
+ * src/raspakit/cbmc/interactions_framework_molecule.cpp:57-68.  kmax3 = ForceField::numberOfWaveVectors;
+ * recip_int_cut2 = reciprocalIntegerCutOffSquared (<= 0: none, the automaticEwald = false default).
+ * capacity_atoms bounds the adsorbate atoms. */
+rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* cell9, int is_rectangular,
+                           const rg_framework* fw, int n_grids, const int* grid_types, rg_grid* const* grids,
+                           const rg_grid* coulomb_grid, const int* kmax3, long long recip_int_cut2,
+                           int capacity_atoms, rg_system** out);
+rg_status rg_system_destroy(rg_system* sys);
+rg_status rg_system_info(const rg_system* sys, int* n_kvectors, double* single_ion_fourier_sum, double* net_charge,
+                         int* n_adsorbate_atoms);
+/* Replaces the adsorbate atoms (pos n*3; molecule ids consecutive per molecule) and rebuilds the stored structure
+ * factor (framework + adsorbates). */
+rg_status rg_system_set_adsorbates(rg_ctx* ctx, rg_system* sys, int n, const double* pos, const int* type,
+                                   const double* charge, const int* molecule);
+
+/* Trial-site energies: CBMC::computeFrameworkMoleculeEnergy + CBMC::computeInterMolecularEnergy for n sites
+ * (src/raspakit/cbmc/interactions.cpp:54-96, interactions_framework_molecule.cpp:31-111,
+ * interactions_intermolecular.cpp:22-95).  Adsorbate atoms of the site's own molecule and of 'skip_molecule'
+ * (skipBackgroundMolecule; -1 for none) are excluded.  energies[4*i + 0..3] = framework VDW, framework Coulomb,
+ * inter-molecular VDW, inter-molecular Coulomb; overlap[i] = 1 where the reference returns std::nullopt. */
+rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* sys, int n, const double* pos, const int* type,
+                            const double* charge, const int* molecule, int skip_molecule, double* energies,
+                            int* overlap);
+
+/* Rosenbluth weight and selection for n_sets sets of k trials (src/raspakit/cbmc/multiple_first_bead.cpp:20-49,
+ * src/raspakit/cbmc/util.cpp:15-48): log_weight[s] = log(sum_i exp(-beta U_i) / k) over non-overlapping trials,
+ * selected[s] = the trial the cumulative search picks for uniforms[s] (-1 if all overlap). */
+rg_status rg_rosenbluth_select(rg_ctx* ctx, int n_sets, int k, const double* energies, const int* overlap,
+                               double beta, const double* uniforms, int* selected, double* log_weight);
+
+/* Interactions::energyDifferenceEwaldFourier (src/raspakit/interactions/ewald.cpp:1043-1244) replacing the old
+ * atoms by the new ones (charges already scaled): out[0] = ewald_fourier (incl. the net-charge term),
+ * out[1] = ewald_self, out[2] = ewald_exclusion.  Writes the trial structure factor; at most 32 atoms. */
+rg_status rg_ewald_delta(rg_ctx* ctx, rg_system* sys, int n_new, const double* new_pos, const double* new_charge,
+                         const int* new_molecule, int n_old, const double* old_pos, const double* old_charge,
+                         const int* old_molecule, double* out);
+/* Interactions::acceptEwaldMove (src/raspakit/interactions/ewald.cpp:1565-1574): stored <- trial. */
+rg_status rg_ewald_accept(rg_ctx* ctx, rg_system* sys);
+/* Interactions::computeEwaldFourierEnergy (src/raspakit/interactions/ewald.cpp:352-563) of the adsorbates from the
+ * stored structure factor: out[0] = fourier (incl. net charge), out[1] = self, out[2] = exclusion. */
+rg_status rg_ewald_energy(rg_ctx* ctx, const rg_system* sys, double* out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Monte Carlo on the device: Widom insertions and multi-system GCMC of rigid molecules
+ * (src/raspakit/mc_moves/component/{widom,insertion_cbmc,deletion_cbmc}.cpp; first beads
+ * src/raspakit/cbmc/multiple_first_bead.cpp; orientations of the rigid body src/raspakit/cbmc/operators.cpp:958-968).
+ * ------------------------------------------------------------------------------------------- */
+
+/* A rigid component (Component, src/raspakit/component.ixx): atoms in a reference geometry; the first bead is the
+ * atom grown first (Component::startingBead). */
+typedef struct rg_component_desc
+{
+  int n_atoms;          /* 1..4 */
+  int first_bead;
+  double position[12];  /* n_atoms * 3, reference geometry (A) */
+  int type[4];          /* pseudo-atom indices */
+  double charge[4];
+  double mol_fraction;  /* Component::molFraction */
+} rg_component_desc;
+
+typedef struct rg_mc_params
+{
+  double temperature;       /* K */
+  double pressure_pa;       /* Pa; fugacity = mole fraction * pressure (fugacity coefficient 1) */
+  int k_first;              /* numberOfFirstBeadPositions (reference default 10), <= 64 */
+  int k_orient;             /* numberOfTrialDirections (reference default 10), <= 64 */
+  double p_translation;     /* move probabilities; the rest are CBMC swaps, half insertions, half deletions */
+  double p_rotation;
+  double max_translation;   /* A */
+  double max_rotation;      /* rad */
+  int min_steps_per_cycle;  /* steps per cycle = max(this, N) (src/raspakit/drivers/monte_carlo.cpp:301) */
+  int max_molecules;
+  unsigned long long seed;  /* Philox4x32-10 key; trials are addressed by (system, step, slot) */
+  long long production_cycles; /* cycles over which the 5 block averages are spread */
+} rg_mc_params;
+
+typedef struct rg_mc_state_t
+{
+  int n_molecules;
+  int n_per_component[2];
+  double energy;            /* running adsorbate energy (internal units) */
+  long long steps, cycles_sampled;
+  long long trials[4], accepted[4];  /* insertion, deletion, translation, rotation */
+  double block_count[5], block_energy[5], block_n[5][2], block_energy_n[5][2], block_nn[5][2][2];
+} rg_mc_state_t;
+
+typedef struct rg_mc rg_mc;
+
+/* Widom insertions count..count+first-1 into a system (MC_Moves::WidomMove, widom.cpp:32-155): per insertion the
+ * log Rosenbluth weight (-inf if every trial overlaps), the external energy of the grown molecule and the Ewald
+ * Fourier energy difference.  W = exp(log_weight - beta ewald) / W_IG with W_IG = 1 for rigid bodies. */
+rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component, const rg_mc_params* params,
+                   long long first, int count, double* log_weight, double* energy, double* ewald);
+
+/* GCMC over n_systems independent, initially empty systems sharing one force field; one wavefront per system. */
+rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, int n_components,
+                       const rg_component_desc* components, const rg_mc_params* params, rg_mc** out);
+/* Hands the final adsorbate count and net charge back to the systems and releases the engine. */
+rg_status rg_mc_destroy(rg_mc* mc);
+/* Learned first-bead proposal for one system: probability[v] > 0 over n3[0]*n3[1]*n3[2] voxels of the cell (x fastest);
+ * NULL restores the uniform proposal. */
+rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability);
+/* Runs 'cycles' MC cycles; sample != 0 accumulates block averages; use_graph != 0 replays one captured cycle. */
+rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);
+rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
+/* Adsorbate atoms of a system (4 slots per molecule; unused slots have type -1) and the component of each molecule. */
+rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacity, double* pos, int* type,
+                           int* component, int* n_out);
+/* Adsorbate energy of every system recomputed from scratch (System::computeTotalEnergies, src/raspakit/system.cpp:690). */
+rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out);
 
 #ifdef __cplusplus
 }

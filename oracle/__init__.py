@@ -273,3 +273,122 @@ def ewald_delta(cell, rect, alpha, kmax, stored, new, old, net_charge, int_cut2=
 
 def select_trial(lbf, uniform):
     return lib().orc_select_trial(len(lbf), _d(_f(lbf)), uniform)
+
+
+def philox4x32(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, np.uint32)
+    up = C.POINTER(C.c_uint32)
+    lib().orc_philox4x32(c.ctypes.data_as(up), k.ctypes.data_as(up), o.ctypes.data_as(up))
+    return o
+
+
+def _mc_lib():
+    L = lib()
+    if not getattr(L, "_mc_ready", False):
+        vp = C.c_void_p
+        L.orc_mc_create.restype = vp
+        L.orc_mc_create.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp, ip, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, C.c_double, C.c_int, dp, ip, dp, C.POINTER(dp), dp, dp, ip,
+                                    ip, C.c_longlong, C.c_int, ip, dp, dp, C.POINTER(C.c_longlong), C.c_ulonglong]
+        L.orc_mc_destroy.argtypes = [vp]
+        L.orc_mc_set_proposal.argtypes = [vp, ip, dp]
+        L.orc_mc_run.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_mc_state.argtypes = [vp, dp, dp]
+        L.orc_mc_adsorbates.argtypes = [vp, dp, ip, ip]
+        L.orc_mc_adsorbates.restype = C.c_int
+        L.orc_widom.argtypes = [vp, C.c_int, C.c_longlong, C.c_int, dp, dp, dp]
+        L._mc_ready = True
+    return L
+
+
+class MC:
+    """One system of the oracle's Monte Carlo replay (same moves and Philox stream layout as the device engine).
+
+    fw: workloads.Framework (direct sums) unless grids = (dict type -> (nz, ny, nx, 8) array, coulomb array, dims)
+    comps: list of dicts (n_atoms, first_bead, position (n,3), type, charge, mol_fraction)
+    params: dict(temperature, pressure_pa, k_first, k_orient, p_translation, p_rotation, max_translation,
+                 max_rotation, min_steps_per_cycle, max_molecules, seed, production_cycles)."""
+
+    def __init__(self, spec, fw, comps, params, kmax=(8, 8, 8), system=0, grids=None, int_cut2=0):
+        L = _mc_lib()
+        pairs = pairs_from_spec(spec)
+        n = pairs["n"]
+        self._keep = []
+        cell9 = _f(np.asarray(fw.cell).T.reshape(9))
+        gptrs = None
+        cg = None
+        gdims = None
+        gcell9 = cell9
+        if grids is not None:
+            gmap, coul, dims = grids[:3]
+            if len(grids) > 3:
+                gcell9 = _f(np.asarray(grids[3]).T.reshape(9))
+            arrs = [None] * n
+            for t, g in gmap.items():
+                arrs[t] = _f(g).reshape(-1)
+            self._keep += [a for a in arrs if a is not None]
+            gptrs = (dp * n)(*[(a.ctypes.data_as(dp) if a is not None else dp()) for a in arrs])
+            if coul is not None:
+                cg = _f(coul).reshape(-1)
+                self._keep.append(cg)
+            gdims = _n(dims)
+            self._keep.append(gdims)
+        ci = np.zeros(6 * len(comps), np.int32)
+        cd = np.zeros(17 * len(comps))
+        for c, d in enumerate(comps):
+            na = d["n_atoms"]
+            ci[6 * c] = na
+            ci[6 * c + 1] = d["first_bead"]
+            ci[6 * c + 2:6 * c + 2 + na] = d["type"]
+            cd[17 * c:17 * c + 3 * na] = np.asarray(d["position"], float).reshape(-1)
+            cd[17 * c + 12:17 * c + 12 + na] = d["charge"]
+            cd[17 * c + 16] = d["mol_fraction"]
+        dpar = _f([params["temperature"], params["pressure_pa"], params["p_translation"], params["p_rotation"],
+                   params["max_translation"], params["max_rotation"]])
+        ipar = (C.c_longlong * 6)(params["k_first"], params["k_orient"], params["min_steps_per_cycle"],
+                                  params["max_molecules"], params["production_cycles"], system)
+        self._keep += [ci, cd, dpar, cell9, gcell9, pairs]
+        fpos, ft, fq = _f(fw.pos), _n(fw.type), _f(fw.charge)
+        self._keep += [fpos, ft, fq]
+        self.h = L.orc_mc_create(_d(cell9), int(fw.rect), n, _d(pairs["eps"]), _d(pairs["sigma"]), _d(pairs["shift"]),
+                                 _i(pairs["lj"]), spec.rc_framework, spec.rc_molecule, spec.rc_coulomb, spec.alpha,
+                                 int(spec.use_charge), spec.overlap, len(ft), _d(fpos), _i(ft), _d(fq), gptrs,
+                                 _d(cg) if cg is not None else None, _d(gcell9),
+                                 _i(gdims) if gdims is not None else None, _i(_n(kmax)), int(int_cut2), len(comps),
+                                 _i(ci), _d(cd), _d(dpar), ipar, int(params["seed"]))
+
+    def set_proposal(self, n3, prob):
+        p = _f(prob).reshape(-1)
+        self._keep.append(p)
+        _mc_lib().orc_mc_set_proposal(self.h, _i(_n(n3)), _d(p))
+
+    def run(self, cycles, sample=False):
+        _mc_lib().orc_mc_run(self.h, int(cycles), int(sample))
+
+    def state(self):
+        s = np.zeros(14)
+        b = np.zeros(50)
+        _mc_lib().orc_mc_state(self.h, _d(s), _d(b))
+        return dict(n_molecules=int(s[0]), n_per_component=[int(s[1]), int(s[2])], energy=s[3], steps=int(s[4]),
+                    cycles=int(s[5]), trials=s[6:10].astype(int), accepted=s[10:14].astype(int),
+                    blocks=b.reshape(5, 10))
+
+    def adsorbates(self, capacity=4096):
+        pos = np.zeros((capacity, 3))
+        t = np.zeros(capacity, np.int32)
+        c = np.zeros(capacity, np.int32)
+        n = _mc_lib().orc_mc_adsorbates(self.h, _d(pos), _i(t), _i(c))
+        return pos[:n], t[:n], c[:n // 4]
+
+    def widom(self, component, first, count):
+        lw, u, ew = np.zeros(count), np.zeros(count), np.zeros(count)
+        _mc_lib().orc_widom(self.h, component, first, count, _d(lw), _d(u), _d(ew))
+        return lw, u, ew
+
+    def __del__(self):
+        try:
+            _mc_lib().orc_mc_destroy(self.h)
+        except Exception:
+            pass

@@ -18,6 +18,8 @@
 //                                                                    polynomials, node values
 //   trial selection                                               -> exact given the uniform
 #include <algorithm>
+#include <array>
+#include <cfloat>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -1160,3 +1162,5 @@ int orc_select_trial(int n, const double* lbf, double uniform)
 }
 
 }  // extern "C"
+
+#include "oracle_mc.inc"

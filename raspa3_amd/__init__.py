@@ -161,6 +161,91 @@ class Grid:
             pass
 
 
+def ewald_parameters(precision, rc, widths):
+    """ForceField::initializeEwaldParameters (automatic): returns (alpha, kmax[3])."""
+    a = C.c_double()
+    k = np.zeros(3, np.int32)
+    L.check(lib().rg_ewald_parameters(precision, rc, L.d(L.f64(widths)), C.byref(a), L.i(k)), None,
+            "rg_ewald_parameters")
+    return a.value, [int(x) for x in k]
+
+
+class System:
+    """rg_system: box, force field, framework (direct) and/or grids, adsorbates, Ewald state."""
+
+    def __init__(self, ctx, ff, cell, rect, framework=None, grids=None, coulomb_grid=None, kmax=(8, 8, 8),
+                 int_cut2=0, capacity=4096):
+        self.ctx = ctx
+        self._keep = (ff, framework, grids, coulomb_grid)
+        grids = grids or {}
+        gtypes = L.i32(list(grids.keys()))
+        garr = (C.c_void_p * max(1, len(grids)))(*[g.h.value for g in grids.values()])
+        h = C.c_void_p()
+        c9 = L.f64(np.asarray(cell).T.reshape(9))
+        L.check(lib().rg_system_create(ctx.h, ff.h, L.d(c9), int(rect), framework.h if framework else None,
+                                       len(grids), L.i(gtypes), garr, coulomb_grid.h if coulomb_grid else None,
+                                       L.i(L.i32(kmax)), int(int_cut2), int(capacity), C.byref(h)),
+                ctx.h, "rg_system_create")
+        self.h = h
+
+    def info(self):
+        nk, n = C.c_int(), C.c_int()
+        si, q = C.c_double(), C.c_double()
+        L.check(lib().rg_system_info(self.h, C.byref(nk), C.byref(si), C.byref(q), C.byref(n)), None, "info")
+        return dict(nk=nk.value, single_ion_sum=si.value, net_charge=q.value, n_adsorbate_atoms=n.value)
+
+    def set_adsorbates(self, pos, type, charge, molecule):
+        pos, t, q, m = L.f64(pos).reshape(-1, 3), L.i32(type), L.f64(charge), L.i32(molecule)
+        L.check(lib().rg_system_set_adsorbates(self.ctx.h, self.h, len(t), L.d(pos), L.i(t), L.d(q), L.i(m)),
+                self.ctx.h, "rg_system_set_adsorbates")
+
+    def trial_energies(self, pos, type, charge, molecule, skip_molecule=-1):
+        pos, t, q, m = L.f64(pos).reshape(-1, 3), L.i32(type), L.f64(charge), L.i32(molecule)
+        e = np.zeros((len(t), 4))
+        ov = np.zeros(len(t), np.int32)
+        L.check(lib().rg_trial_energies(self.ctx.h, self.h, len(t), L.d(pos), L.i(t), L.d(q), L.i(m), skip_molecule,
+                                        L.d(e), L.i(ov)), self.ctx.h, "rg_trial_energies")
+        return e, ov.astype(bool)
+
+    def ewald_delta(self, new_pos, new_q, new_mol, old_pos, old_q, old_mol):
+        def arr(p, q, m):
+            return L.f64(p).reshape(-1, 3), L.f64(q), L.i32(m)
+        npos, nq, nm = arr(new_pos, new_q, new_mol)
+        opos, oq, om = arr(old_pos, old_q, old_mol)
+        out = np.zeros(3)
+        L.check(lib().rg_ewald_delta(self.ctx.h, self.h, len(nq), L.d(npos), L.d(nq), L.i(nm), len(oq), L.d(opos),
+                                     L.d(oq), L.i(om), L.d(out)), self.ctx.h, "rg_ewald_delta")
+        return out
+
+    def ewald_accept(self):
+        L.check(lib().rg_ewald_accept(self.ctx.h, self.h), self.ctx.h, "rg_ewald_accept")
+
+    def ewald_energy(self):
+        out = np.zeros(3)
+        L.check(lib().rg_ewald_energy(self.ctx.h, self.h, L.d(out)), self.ctx.h, "rg_ewald_energy")
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().rg_system_destroy(self.h)
+        except Exception:
+            pass
+
+
+def rosenbluth_select(ctx, energies, overlap, beta, uniforms):
+    """energies, overlap: (n_sets, k).  Returns (selected[n_sets], log_weight[n_sets])."""
+    U = L.f64(energies)
+    ns, k = U.shape
+    ov = L.i32(overlap)
+    u = L.f64(uniforms)
+    sel = np.zeros(ns, np.int32)
+    lw = np.zeros(ns)
+    L.check(lib().rg_rosenbluth_select(ctx.h, ns, k, L.d(U), L.i(ov), beta, L.d(u), L.i(sel), L.d(lw)), ctx.h,
+            "rg_rosenbluth_select")
+    return sel, lw
+
+
 def build_grids(ctx, ff, fw, specs, sync=True):
     """Builds several equal-dimension grids in one fused pass.  specs: list of (Grid, type, probe, cutoff)."""
     n = len(specs)
@@ -170,3 +255,94 @@ def build_grids(ctx, ff, fw, specs, sync=True):
     arr = (C.c_void_p * n)(*[s[0].h.value for s in specs])
     fn = lib().rg_grid_build_many if sync else lib().rg_grid_build_many_async
     L.check(fn(ctx.h, ff.h, fw.h, n, L.i(types), L.i(probes), L.d(cuts), arr), ctx.h, "rg_grid_build_many")
+
+
+def component_desc(d):
+    """rg_component_desc from a dict(n_atoms, first_bead, position (n,3), type, charge, mol_fraction)."""
+    c = L.ComponentDesc()
+    c.n_atoms = d["n_atoms"]
+    c.first_bead = d["first_bead"]
+    pos = np.asarray(d["position"], float).reshape(-1)
+    for k in range(len(pos)):
+        c.position[k] = pos[k]
+    for a in range(d["n_atoms"]):
+        c.type[a] = int(d["type"][a])
+        c.charge[a] = float(d["charge"][a])
+    c.mol_fraction = d.get("mol_fraction", 1.0)
+    return c
+
+
+def mc_params(p):
+    q = L.McParams()
+    for name, _ in L.McParams._fields_:
+        setattr(q, name, p[name])
+    return q
+
+
+def widom(ctx, system, comp, params, first, count):
+    """rg_widom: returns (log_weight, energy, ewald) per insertion."""
+    lw, u, ew = np.zeros(count), np.zeros(count), np.zeros(count)
+    c = component_desc(comp)
+    p = mc_params(params)
+    L.check(lib().rg_widom(ctx.h, system.h, C.byref(c), C.byref(p), int(first), int(count), L.d(lw), L.d(u), L.d(ew)),
+            ctx.h, "rg_widom")
+    return lw, u, ew
+
+
+class Mc:
+    """rg_mc: GCMC over several systems (one wavefront each)."""
+
+    def __init__(self, ctx, systems, comps, params):
+        self.ctx = ctx
+        self.systems = systems
+        arr = (C.c_void_p * len(systems))(*[s.h.value for s in systems])
+        descs = (L.ComponentDesc * len(comps))(*[component_desc(c) for c in comps])
+        p = mc_params(params)
+        h = C.c_void_p()
+        L.check(lib().rg_mc_create(ctx.h, len(systems), arr, len(comps), descs, C.byref(p), C.byref(h)), ctx.h,
+                "rg_mc_create")
+        self.h = h
+
+    def set_proposal(self, system, n3, prob):
+        n = L.i32(n3)
+        p = None if prob is None else L.f64(prob).reshape(-1)
+        L.check(lib().rg_mc_set_proposal(self.ctx.h, self.h, system, L.i(n), L.d(p)), self.ctx.h, "set_proposal")
+
+    def run(self, cycles, sample=False, graph=True):
+        L.check(lib().rg_mc_run(self.ctx.h, self.h, int(cycles), int(sample), int(graph)), self.ctx.h, "rg_mc_run")
+
+    def state(self, system):
+        s = L.McState()
+        L.check(lib().rg_mc_state(self.ctx.h, self.h, system, C.byref(s)), self.ctx.h, "rg_mc_state")
+        return dict(n_molecules=s.n_molecules, n_per_component=list(s.n_per_component), energy=s.energy,
+                    steps=s.steps, cycles=s.cycles_sampled, trials=np.array(list(s.trials)),
+                    accepted=np.array(list(s.accepted)),
+                    block_count=np.array(list(s.block_count)), block_energy=np.array(list(s.block_energy)),
+                    block_n=np.array([list(r) for r in s.block_n]),
+                    block_energy_n=np.array([list(r) for r in s.block_energy_n]),
+                    block_nn=np.array([[list(q) for q in r] for r in s.block_nn]))
+
+    def adsorbates(self, system, capacity=1 << 16):
+        pos = np.zeros((capacity, 3))
+        t = np.zeros(capacity, np.int32)
+        c = np.zeros(capacity, np.int32)
+        n = C.c_int()
+        L.check(lib().rg_mc_adsorbates(self.ctx.h, self.h, system, capacity, L.d(pos), L.i(t), L.i(c), C.byref(n)),
+                self.ctx.h, "rg_mc_adsorbates")
+        return pos[:n.value], t[:n.value], c[:n.value // 4]
+
+    def total_energy(self):
+        out = np.zeros(len(self.systems))
+        L.check(lib().rg_mc_total_energy(self.ctx.h, self.h, L.d(out)), self.ctx.h, "rg_mc_total_energy")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().rg_mc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

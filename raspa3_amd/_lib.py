@@ -47,6 +47,17 @@ SIGNATURES = {
     "rg_grid_download": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
     "rg_grid_upload": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
     "rg_grid_interpolate": (C.c_int, [vp, vp, C.c_int, dp, dp, dp]),
+    "rg_ewald_parameters": (C.c_int, [C.c_double, C.c_double, dp, dp, ip]),
+    "rg_system_create": (C.c_int, [vp, vp, dp, C.c_int, vp, C.c_int, ip, C.POINTER(vp), vp, ip, C.c_longlong,
+                                   C.c_int, C.POINTER(vp)]),
+    "rg_system_destroy": (C.c_int, [vp]),
+    "rg_system_info": (C.c_int, [vp, ip, dp, dp, ip]),
+    "rg_system_set_adsorbates": (C.c_int, [vp, vp, C.c_int, dp, ip, dp, ip]),
+    "rg_trial_energies": (C.c_int, [vp, vp, C.c_int, dp, ip, dp, ip, C.c_int, dp, ip]),
+    "rg_rosenbluth_select": (C.c_int, [vp, C.c_int, C.c_int, dp, ip, C.c_double, dp, ip, dp]),
+    "rg_ewald_delta": (C.c_int, [vp, vp, C.c_int, dp, dp, ip, C.c_int, dp, dp, ip, dp]),
+    "rg_ewald_accept": (C.c_int, [vp, vp]),
+    "rg_ewald_energy": (C.c_int, [vp, vp, dp]),
 }
 
 
@@ -88,3 +99,37 @@ def check(status, ctx=None, what=""):
     if status != RG_OK:
         msg = lib().rg_last_error(ctx).decode() if ctx else ""
         raise RGError(f"{what}: status {status}: {msg}")
+
+
+class ComponentDesc(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("first_bead", C.c_int), ("position", C.c_double * 12),
+                ("type", C.c_int * 4), ("charge", C.c_double * 4), ("mol_fraction", C.c_double)]
+
+
+class McParams(C.Structure):
+    _fields_ = [("temperature", C.c_double), ("pressure_pa", C.c_double), ("k_first", C.c_int),
+                ("k_orient", C.c_int), ("p_translation", C.c_double), ("p_rotation", C.c_double),
+                ("max_translation", C.c_double), ("max_rotation", C.c_double), ("min_steps_per_cycle", C.c_int),
+                ("max_molecules", C.c_int), ("seed", C.c_ulonglong), ("production_cycles", C.c_longlong)]
+
+
+class McState(C.Structure):
+    _fields_ = [("n_molecules", C.c_int), ("n_per_component", C.c_int * 2), ("energy", C.c_double),
+                ("steps", C.c_longlong), ("cycles_sampled", C.c_longlong), ("trials", C.c_longlong * 4),
+                ("accepted", C.c_longlong * 4), ("block_count", C.c_double * 5), ("block_energy", C.c_double * 5),
+                ("block_n", (C.c_double * 2) * 5), ("block_energy_n", (C.c_double * 2) * 5),
+                ("block_nn", ((C.c_double * 2) * 2) * 5)]
+
+
+SIGNATURES.update({
+    "rg_widom": (C.c_int, [vp, vp, C.POINTER(ComponentDesc), C.POINTER(McParams), C.c_longlong, C.c_int, dp, dp,
+                           dp]),
+    "rg_mc_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(ComponentDesc), C.POINTER(McParams),
+                               C.POINTER(vp)]),
+    "rg_mc_destroy": (C.c_int, [vp]),
+    "rg_mc_set_proposal": (C.c_int, [vp, vp, C.c_int, ip, dp]),
+    "rg_mc_run": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
+    "rg_mc_state": (C.c_int, [vp, vp, C.c_int, C.POINTER(McState)]),
+    "rg_mc_adsorbates": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, ip, ip, ip]),
+    "rg_mc_total_energy": (C.c_int, [vp, vp, dp]),
+})

@@ -1,0 +1,822 @@
+// Device-resident Monte Carlo for rigid molecules on gfx950: Widom insertion batches and multi-system GCMC.
+//
+// One wavefront owns one system (GCMC) or one insertion (Widom); four per workgroup.  Inside a move each lane takes
+// one trial (first-bead position or orientation): grid-typed sites are interpolated lane-parallel, direct sums and
+// inter-molecular sums loop cooperatively over atoms with wavefront-shuffle reductions, and the Ewald Fourier
+// difference strides over k-vectors with FP64 eik recursions per atom in LDS.
+//
+// Moves (src/raspakit/mc_moves/component/): Widom (widom.cpp:32-155), CBMC swap insertion (insertion_cbmc.cpp:34-200)
+// and deletion (deletion_cbmc.cpp:67-174) with multiple first beads (src/raspakit/cbmc/multiple_first_bead.cpp:20-76)
+// followed by uniformly random orientations of the rigid body about the first bead (the rigid seed step of
+// src/raspakit/cbmc/operators.cpp:958-968), translation and rotation.  The ideal-gas Rosenbluth weight of a rigid
+// body with orientation trials is 1.  Trials come from a Philox4x32-10 stream addressed by (system, move, slot),
+// which the oracle replays.
+//
+// A learned insertion proposal replaces the uniform first-bead distribution by a tabulated density q on voxels
+// (sampled through an alias table, uniformly inside the voxel); each first-bead trial then carries the weight
+// exp(-beta U) / (V q) in both growth and retrace, so the acceptance rule stays exact.
+#include <cmath>
+#include <cstring>
+
+#include "mc_engine.hpp"
+
+namespace rg
+{
+
+// ---------------------------------------------------------------------------------------------------------------
+// random numbers: slot layout of one move
+// ---------------------------------------------------------------------------------------------------------------
+enum : uint32_t
+{
+  SLOT_MOVE = 0,       // (move type, component)
+  SLOT_PICK = 1,       // (molecule, acceptance)
+  SLOT_SELECT = 2,     // (first-bead selection, orientation selection)
+  SLOT_DISP = 3,       // translation (2 slots) / rotation (2 slots)
+  SLOT_FIRST = 16,     // first-bead trial j: slots 16 + 2j, 17 + 2j
+  SLOT_ORIENT = 160,   // orientation trial j: slots 160 + 2j, 161 + 2j
+  SLOT_VOXEL = 304     // proposal trial j: slots 304 + 2j, 305 + 2j
+};
+
+__device__ __forceinline__ void uni(const McRun& R, int sys, unsigned long long step, uint32_t slot, double& a,
+                                    double& b)
+{
+  rgd::uniform2(R.seed, uint32_t(sys), uint32_t(step), uint32_t(step >> 32), slot, a, b);
+}
+
+// Ordering inside one wavefront: LDS (compiler barrier + wavefront fence) and global memory written by one lane
+// and read by others (workgroup-scope fence: waits for the stores and keeps the per-CU cache coherent).
+__device__ __forceinline__ void waveSyncLds()
+{
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void waveSyncGlobal()
+{
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Rotation matrix (row-major) of the unit quaternion from three uniforms (K. Shoemake, "Uniform random rotations",
+// Graphics Gems III, 1992).
+__device__ __forceinline__ void randomRotation(double u1, double u2, double u3, double Rm[9])
+{
+  const double s1 = sqrt(1.0 - u1), s2 = sqrt(u1);
+  double sa, ca, sb, cb;
+  sincos(2.0 * M_PI * u2, &sa, &ca);
+  sincos(2.0 * M_PI * u3, &sb, &cb);
+  const double x = s1 * sa, y = s1 * ca, z = s2 * sb, w = s2 * cb;
+  Rm[0] = 1.0 - 2.0 * (y * y + z * z);
+  Rm[1] = 2.0 * (x * y - z * w);
+  Rm[2] = 2.0 * (x * z + y * w);
+  Rm[3] = 2.0 * (x * y + z * w);
+  Rm[4] = 1.0 - 2.0 * (x * x + z * z);
+  Rm[5] = 2.0 * (y * z - x * w);
+  Rm[6] = 2.0 * (x * z - y * w);
+  Rm[7] = 2.0 * (y * z + x * w);
+  Rm[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// energies of sets of sites
+// ---------------------------------------------------------------------------------------------------------------
+
+// External energy of one site with the whole wavefront (framework direct or grid, plus adsorbates except molecule
+// 'own' and 'skip'); overlap as in the reference (any pair above the criterion, or the grid value above it).
+__device__ double siteExternal(const SysDev& S, int nads, double px, double py, double pz, int type, double q, int own,
+                               int skip, int lane, bool& ov)
+{
+  double e = 0.0;
+  bool o = false;
+  const int gi = S.gridOfType ? S.gridOfType[type] : -1;
+  if (gi < 0)
+  {
+    for (int a = lane; a < S.nfw; a += 64)
+    {
+      double dx = S.fwx[a] - px, dy = S.fwy[a] - py, dz = S.fwz[a] - pz;
+      rgd::pbc(S.box, dx, dy, dz);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < S.rcv2Fw)
+      {
+        const double v = rgd::vdw0(S.pairs[S.fwt[a] * S.ntypes + type], 1.0, rr);
+        o = o || v > S.overlap;
+        e += v;
+      }
+      if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.fwq[a] * q);
+    }
+  }
+  for (int a = lane; a < nads; a += 64)
+  {
+    const int m = S.amol[a];
+    const int t = S.at[a];
+    if (m == own || m == skip || t < 0) continue;
+    double dx = S.ax[a] - px, dy = S.ay[a] - py, dz = S.az[a] - pz;
+    rgd::pbc(S.box, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < S.rcv2Mol)
+    {
+      const double v = rgd::vdw0(S.pairs[t * S.ntypes + type], 1.0, rr);
+      o = o || v > S.overlap;
+      e += v;
+    }
+    if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.aq[a] * q);
+  }
+  e = rgd::waveSum(e);
+  if (gi >= 0)
+  {
+    const double g = rgd::interpolate(S.grids[gi], px, py, pz);
+    o = o || g > S.overlap;
+    e += g;
+    if (S.useCharge && S.coulGrid >= 0) e += q * rgd::interpolate(S.grids[S.coulGrid], px, py, pz);
+  }
+  ov = __ballot(o) != 0ull;
+  return e;
+}
+
+// Energies of K trial sets of n sites each (sites in LDS as x, y, z per site, set-major).  Lane j < K receives the
+// total of set j and its overlap flag.
+__device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const int* atomIdx, int n, int K,
+                                 const double* pos, int own, int skip, int lane, double& Uj, bool& ovj)
+{
+  Uj = 0.0;
+  ovj = false;
+  for (int j = 0; j < K; ++j)
+  {
+    double u = 0.0;
+    bool any = false;
+    for (int s = 0; s < n; ++s)
+    {
+      const int a = atomIdx[s];
+      const double* p = pos + 3 * (j * n + s);
+      bool o;
+      u += siteExternal(S, nads, p[0], p[1], p[2], c.type[a], c.q[a], own, skip, lane, o);
+      any = any || o;
+    }
+    if (lane == j)
+    {
+      Uj = u;
+      ovj = any;
+    }
+  }
+}
+
+// Rosenbluth weight over the lanes (lane j holds trial j): logW = log(sum_j w_j / K) with w_j = exp(-beta U_j) * bias_j
+// over non-overlapping trials, and the trial the cumulative search picks for uniform u (-1 if none).
+__device__ void waveRosenbluth(double Uj, bool ovj, double logBias, int K, double beta, double u, int lane, int& sel,
+                               double& logW)
+{
+  const bool valid = lane < K && !ovj;
+  const double lbf = valid ? -beta * Uj + logBias : -INFINITY;
+  const double mx = rgd::waveMax(lbf);
+  if (__ballot(valid) == 0ull)
+  {
+    sel = -1;
+    logW = -INFINITY;
+    return;
+  }
+  const double w = valid ? exp(lbf - mx) : 0.0;
+  // inclusive prefix sum in lane order (the reference's accumulation order)
+  double pre = w;
+  for (int o = 1; o < 64; o <<= 1)
+  {
+    const double t = __shfl_up(pre, o, 64);
+    if (lane >= o) pre += t;
+  }
+  const double sum = __shfl(pre, 63, 64);
+  const double ws = u * sum;
+  // first valid trial whose cumulative weight reaches ws; else the last valid trial
+  const unsigned long long reach = __ballot(valid && !(pre < ws));
+  const unsigned long long vmask = __ballot(valid);
+  sel = reach ? __ffsll((long long)reach) - 1 : 63 - __clzll((long long)vmask);
+  logW = mx + log(sum) - log(double(K));
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Ewald Fourier difference with one wavefront
+// ---------------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// atoms: x, y, z, q (q carries the sign: + new, - old); tab: per atom 3 x (EW_MAXK + 1) complex.
+__device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, double2* tab, int lane, bool writeTrial)
+{
+  const int stride = 3 * (EW_MAXK + 1);
+  for (int t = lane; t < na * 3; t += 64)
+  {
+    const int a = t / 3, ax = t % 3;
+    const double s =
+        2.0 * M_PI * (S.box.inv[ax] * at[4 * a] + S.box.inv[3 + ax] * at[4 * a + 1] + S.box.inv[6 + ax] * at[4 * a + 2]);
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    const double2 e1 = make_double2(cs, sn);
+    double2* row = tab + a * stride + ax * (EW_MAXK + 1);
+    row[0] = make_double2(1.0, 0.0);
+    double2 v = e1;
+    for (int k = 1; k <= S.kmax[ax]; ++k)
+    {
+      row[k] = v;
+      v = cmul(v, e1);
+    }
+  }
+  waveSyncLds();
+  double de = 0.0;
+  for (int k = lane; k < S.nk; k += 64)
+  {
+    const int code = S.kcode[k];
+    const int kx = code & 0xff, ky = ((code >> 8) & 0xff) - 128, kz = ((code >> 16) & 0xff) - 128;
+    double2 d = make_double2(0.0, 0.0);
+    for (int a = 0; a < na; ++a)
+    {
+      const double2* row = tab + a * stride;
+      const double2 ex = row[kx];
+      double2 ey = row[(EW_MAXK + 1) + (ky < 0 ? -ky : ky)];
+      if (ky < 0) ey.y = -ey.y;
+      double2 ez = row[2 * (EW_MAXK + 1) + (kz < 0 ? -kz : kz)];
+      if (kz < 0) ez.y = -ez.y;
+      const double2 e = cmul(cmul(ex, ey), ez);
+      d.x += at[4 * a + 3] * e.x;
+      d.y += at[4 * a + 3] * e.y;
+    }
+    const double2 s = S.Sk[k];
+    const double2 t = make_double2(s.x + d.x, s.y + d.y);
+    de += S.fk[k] * ((t.x * t.x + t.y * t.y) - (s.x * s.x + s.y * s.y));
+    if (writeTrial) S.St[k] = t;
+  }
+  return rgd::waveSum(de);
+}
+
+__device__ void waveAcceptEwald(const SysDev& S, int lane)
+{
+  for (int k = lane; k < S.nk; k += 64) S.Sk[k] = S.St[k];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// insertion growth and deletion retrace (rigid molecules)
+// ---------------------------------------------------------------------------------------------------------------
+
+struct WaveScratch
+{
+  double pos[64 * 3 * (MC_MAXATOMS - 1) > 64 * 3 ? 64 * 3 * (MC_MAXATOMS - 1) : 64 * 3];
+  double at[4 * 2 * MC_MAXATOMS];
+  double2 tab[2 * MC_MAXATOMS * 3 * (EW_MAXK + 1)];
+  double mol[3 * MC_MAXATOMS];
+};
+
+// First-bead position of trial j: uniform in the cell or drawn from the proposal (alias table over voxels, uniform
+// inside the voxel).  Returns log(1 / (V q)) = -log(nvox p_v) (0 for the uniform distribution).
+__device__ double firstBeadTrial(const McRun& R, const SysDev& S, const McSys& Y, int sys, unsigned long long step,
+                                 int j, double p[3])
+{
+  double u0, u1, u2, u3;
+  if (Y.prop.nvox > 0)
+  {
+    const Proposal& P = Y.prop;
+    uni(R, sys, step, SLOT_VOXEL + 2 * j, u0, u1);
+    uni(R, sys, step, SLOT_VOXEL + 2 * j + 1, u2, u3);
+    int col = min(int(u0 * P.nvox), P.nvox - 1);
+    const int v = (u1 < P.aliasProb[col]) ? col : P.alias[col];
+    const int vx = v % P.n[0], vy = (v / P.n[0]) % P.n[1], vz = v / (P.n[0] * P.n[1]);
+    double w0, w1;
+    uni(R, sys, step, SLOT_FIRST + 2 * j, w0, w1);
+    const double f[3] = {(vx + u2) / P.n[0], (vy + u3) / P.n[1], (vz + w0) / P.n[2]};
+    for (int c = 0; c < 3; ++c) p[c] = S.box.cell[c] * f[0] + S.box.cell[3 + c] * f[1] + S.box.cell[6 + c] * f[2];
+    return -P.logNvoxP[v];
+  }
+  uni(R, sys, step, SLOT_FIRST + 2 * j, u0, u1);
+  uni(R, sys, step, SLOT_FIRST + 2 * j + 1, u2, u3);
+  // SimulationBox::randomPosition: cell * (u, u, u) (src/raspakit/simulationbox.cpp:132-135)
+  for (int c = 0; c < 3; ++c) p[c] = S.box.cell[c] * u0 + S.box.cell[3 + c] * u1 + S.box.cell[6 + c] * u2;
+  return 0.0;
+}
+
+__device__ double proposalLogBias(const McSys& Y, const SysDev& S, const double p[3])
+{
+  if (Y.prop.nvox <= 0) return 0.0;
+  const Proposal& P = Y.prop;
+  int idx[3];
+  for (int a = 0; a < 3; ++a)
+  {
+    double f = S.box.inv[a] * p[0] + S.box.inv[3 + a] * p[1] + S.box.inv[6 + a] * p[2];
+    f -= floor(f);
+    idx[a] = min(int(f * P.n[a]), P.n[a] - 1);
+  }
+  return -P.logNvoxP[idx[0] + P.n[0] * (idx[1] + P.n[1] * idx[2])];
+}
+
+// Grows a molecule of component c: K first-bead trials, then (for more than one atom) K orientation trials about
+// the selected first bead.  On success writes the atom positions into W.mol and returns true with the log Rosenbluth
+// weight and the external energy of the grown molecule.  'retrace' >= 0 retraces that existing molecule instead
+// (trial 0 = its current first bead and orientation; its own atoms are excluded).
+__device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
+                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+{
+  const int K1 = R.kFirst, K2 = R.kOrient;
+  const int own = retrace >= 0 ? retrace : -2;
+  const int fb = c.first;
+  // --- first bead
+  double p[3] = {0.0, 0.0, 0.0};
+  double logBias = 0.0;
+  if (lane < K1)
+  {
+    if (retrace >= 0 && lane == 0)
+    {
+      const int a0 = retrace * MC_MAXATOMS + fb;
+      p[0] = S.ax[a0];
+      p[1] = S.ay[a0];
+      p[2] = S.az[a0];
+      logBias = proposalLogBias(Y, S, p);
+    }
+    else
+      logBias = firstBeadTrial(R, S, Y, sys, step, lane, p);
+    W.pos[3 * lane] = p[0];
+    W.pos[3 * lane + 1] = p[1];
+    W.pos[3 * lane + 2] = p[2];
+  }
+  waveSyncLds();
+  double U1;
+  bool ov1;
+  const int fbIdx[1] = {fb};
+  trialSetEnergies(S, nads, c, fbIdx, 1, K1, W.pos, own, -2, lane, U1, ov1);
+  double uSel1, uSel2;
+  uni(R, sys, step, SLOT_SELECT, uSel1, uSel2);
+  int sel1;
+  double logW1;
+  waveRosenbluth(U1, ov1, logBias, K1, R.beta, uSel1, lane, sel1, logW1);
+  if (retrace >= 0) sel1 = 0;  // the retrace keeps the existing molecule
+  if (sel1 < 0 || logW1 < R.logMinW) return false;
+  const double fx = __shfl(p[0], sel1, 64), fy = __shfl(p[1], sel1, 64), fz = __shfl(p[2], sel1, 64);
+  const double Ufirst = __shfl(U1, sel1, 64);
+  if (retrace >= 0 && __shfl(ov1 ? 1 : 0, 0, 64)) return false;
+  if (c.na == 1)
+  {
+    if (lane == 0)
+    {
+      W.mol[0] = fx;
+      W.mol[1] = fy;
+      W.mol[2] = fz;
+    }
+    logW = logW1;
+    Uext = Ufirst;
+    return true;
+  }
+  // --- orientations about the first bead
+  int idx[MC_MAXATOMS - 1];
+  int n = 0;
+  for (int a = 0; a < c.na; ++a)
+    if (a != fb) idx[n++] = a;
+  if (lane < K2)
+  {
+    double Rm[9];
+    if (retrace >= 0 && lane == 0)
+    {
+      for (int s = 0; s < n; ++s)
+      {
+        const int a0 = retrace * MC_MAXATOMS + idx[s];
+        W.pos[3 * s] = S.ax[a0];
+        W.pos[3 * s + 1] = S.ay[a0];
+        W.pos[3 * s + 2] = S.az[a0];
+      }
+    }
+    else
+    {
+      double u1, u2, u3, u4;
+      uni(R, sys, step, SLOT_ORIENT + 2 * lane, u1, u2);
+      uni(R, sys, step, SLOT_ORIENT + 2 * lane + 1, u3, u4);
+      randomRotation(u1, u2, u3, Rm);
+      for (int s = 0; s < n; ++s)
+      {
+        const double* r = c.ref[idx[s]];
+        W.pos[3 * (lane * n + s)] = fx + Rm[0] * r[0] + Rm[1] * r[1] + Rm[2] * r[2];
+        W.pos[3 * (lane * n + s) + 1] = fy + Rm[3] * r[0] + Rm[4] * r[1] + Rm[5] * r[2];
+        W.pos[3 * (lane * n + s) + 2] = fz + Rm[6] * r[0] + Rm[7] * r[1] + Rm[8] * r[2];
+      }
+    }
+  }
+  waveSyncLds();
+  double U2;
+  bool ov2;
+  trialSetEnergies(S, nads, c, idx, n, K2, W.pos, own, -2, lane, U2, ov2);
+  int sel2;
+  double logW2;
+  waveRosenbluth(U2, ov2, 0.0, K2, R.beta, uSel2, lane, sel2, logW2);
+  if (retrace >= 0) sel2 = 0;
+  if (sel2 < 0) return false;
+  if (retrace >= 0 && __shfl(ov2 ? 1 : 0, 0, 64)) return false;
+  const double Uorient = __shfl(U2, sel2, 64);
+  if (lane == 0)
+  {
+    W.mol[3 * fb] = fx;
+    W.mol[3 * fb + 1] = fy;
+    W.mol[3 * fb + 2] = fz;
+    for (int s = 0; s < n; ++s)
+      for (int d = 0; d < 3; ++d) W.mol[3 * idx[s] + d] = W.pos[3 * (sel2 * n + s) + d];
+  }
+  waveSyncLds();
+  logW = logW1 + logW2;
+  Uext = Ufirst + Uorient;
+  return true;
+}
+
+// Ewald energy change of inserting (+1) or removing (-1) the atoms in W.mol of component c, incl. the constant
+// self and intra-molecular exclusion terms of the rigid body.
+__device__ double ewaldInsertRemove(const McRun& R, const SysDev& S, double netCharge, const Comp& c, int sign, int lane,
+                                    WaveScratch& W, bool writeTrial)
+{
+  if (!S.useCharge || !c.charged) return 0.0;
+  if (lane < c.na)
+  {
+    W.at[4 * lane] = W.mol[3 * lane];
+    W.at[4 * lane + 1] = W.mol[3 * lane + 1];
+    W.at[4 * lane + 2] = W.mol[3 * lane + 2];
+    W.at[4 * lane + 3] = sign * c.q[lane];
+  }
+  waveSyncLds();
+  const double df = waveEwaldDelta(S, c.na, W.at, W.tab, lane, writeTrial);
+  // net charge: neutral molecules leave it unchanged; otherwise (Q + dq)^2 - Q^2 times the single-ion term
+  const double dq = sign * c.qsum;
+  const double uIon = -(S.singleIon - S.coulomb * S.alpha / sqrt(M_PI));
+  const double qn = netCharge + dq;
+  return df + uIon * (qn * qn - netCharge * netCharge) - sign * c.eIntra;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Widom insertions: one wavefront per insertion i (the "step" of the stream), system state unchanged.
+// ---------------------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, const SysDev* __restrict__ Sp,
+                                               const McSys* __restrict__ Yp, int comp, long long first, int count,
+                                               double* outLogW, double* outU, double* outEwald)
+{
+  __shared__ WaveScratch scratch[4];
+  const McRun& R = *Rp;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + w;
+  if (i >= count) return;
+  const SysDev& S = *Sp;
+  const McSys& Y = *Yp;
+  const Comp& c = R.comp[comp];
+  double logW = -INFINITY, U = 0.0, de = 0.0;
+  if (growRigid(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
+    de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], false);
+  if (lane == 0)
+  {
+    outLogW[i] = logW;
+    outU[i] = U;
+    outEwald[i] = de;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// GCMC: one wavefront per system, 'cycles' MC cycles per launch.  The wave-uniform state lives in registers (every
+// lane holds the same values) and is written back once by lane 0; the adsorbate arrays, the molecule component
+// table and the structure factors stay in global memory and are ordered with workgroup-scope fences.
+// ---------------------------------------------------------------------------------------------------------------
+struct Live
+{
+  int nmol;
+  int ncomp[MC_MAXCOMP];
+  int nads;
+  double U;
+  double netCharge;
+  long long step;
+  long long cycle;
+  long long trials[4];
+  long long accepted[4];
+};
+
+__device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W)
+{
+  const unsigned long long step = (unsigned long long)L.step;
+  double uMove, uComp, uMol, uAcc;
+  uni(R, sys, step, SLOT_MOVE, uMove, uComp);
+  uni(R, sys, step, SLOT_PICK, uMol, uAcc);
+  int ci = R.ncomp - 1;
+  {
+    double cum = 0.0;
+    for (int q = 0; q < R.ncomp; ++q)
+    {
+      cum += R.comp[q].molFraction;
+      if (uComp < cum)
+      {
+        ci = q;
+        break;
+      }
+    }
+  }
+  const Comp& c = R.comp[ci];
+  const double pTrans = R.pTranslation, pRot = pTrans + R.pRotation;
+  if (uMove < pRot)
+  {
+    // translation or rotation of a random molecule (any component)
+    const bool isTrans = uMove < pTrans;
+    L.trials[isTrans ? 2 : 3] += 1;
+    if (L.nmol == 0) return;
+    const int m = min(int(uMol * L.nmol), L.nmol - 1);
+    const Comp& cm = R.comp[Y.comp[m]];
+    double d0, d1, d2, d3;
+    uni(R, sys, step, SLOT_DISP, d0, d1);
+    uni(R, sys, step, SLOT_DISP + 1, d2, d3);
+    // old positions into W.pos[0 .. na), new ones into W.pos[na .. 2na)
+    if (lane < cm.na)
+    {
+      const int a0 = m * MC_MAXATOMS + lane;
+      const double ox = S.ax[a0], oy = S.ay[a0], oz = S.az[a0];
+      W.pos[3 * lane] = ox;
+      W.pos[3 * lane + 1] = oy;
+      W.pos[3 * lane + 2] = oz;
+      double nx, ny, nz;
+      if (isTrans)
+      {
+        nx = ox + R.maxTranslation * (2.0 * d0 - 1.0);
+        ny = oy + R.maxTranslation * (2.0 * d1 - 1.0);
+        nz = oz + R.maxTranslation * (2.0 * d2 - 1.0);
+      }
+      else
+      {
+        // rotation by theta about a random unit axis through the first bead (Rodrigues' formula)
+        const int f0 = m * MC_MAXATOMS + cm.first;
+        const double cx = S.ax[f0], cy = S.ay[f0], cz = S.az[f0];
+        const double zc = 2.0 * d0 - 1.0, ph = 2.0 * M_PI * d1, sr = sqrt(fmax(0.0, 1.0 - zc * zc));
+        const double kx = sr * cos(ph), ky = sr * sin(ph), kz = zc;
+        const double th = R.maxRotation * (2.0 * d2 - 1.0);
+        const double ct = cos(th), st = sin(th);
+        const double vx = ox - cx, vy = oy - cy, vz = oz - cz;
+        const double kv = kx * vx + ky * vy + kz * vz;
+        nx = cx + vx * ct + (ky * vz - kz * vy) * st + kx * kv * (1.0 - ct);
+        ny = cy + vy * ct + (kz * vx - kx * vz) * st + ky * kv * (1.0 - ct);
+        nz = cz + vz * ct + (kx * vy - ky * vx) * st + kz * kv * (1.0 - ct);
+      }
+      W.pos[3 * (cm.na + lane)] = nx;
+      W.pos[3 * (cm.na + lane) + 1] = ny;
+      W.pos[3 * (cm.na + lane) + 2] = nz;
+    }
+    waveSyncLds();
+    double Uold = 0.0, Unew = 0.0;
+    bool ovNew = false;
+    for (int a = 0; a < cm.na; ++a)
+    {
+      bool o;
+      Uold += siteExternal(S, L.nads, W.pos[3 * a], W.pos[3 * a + 1], W.pos[3 * a + 2], cm.type[a], cm.q[a], m, -2,
+                           lane, o);
+      Unew += siteExternal(S, L.nads, W.pos[3 * (cm.na + a)], W.pos[3 * (cm.na + a) + 1], W.pos[3 * (cm.na + a) + 2],
+                           cm.type[a], cm.q[a], m, -2, lane, o);
+      ovNew = ovNew || o;
+    }
+    if (ovNew) return;
+    double de = 0.0;
+    const bool ew = S.useCharge && cm.charged;
+    if (ew)
+    {
+      if (lane < 2 * cm.na)
+      {
+        const bool isNew = lane >= cm.na;
+        const int a = isNew ? lane - cm.na : lane;
+        W.at[4 * lane] = W.pos[3 * lane];
+        W.at[4 * lane + 1] = W.pos[3 * lane + 1];
+        W.at[4 * lane + 2] = W.pos[3 * lane + 2];
+        W.at[4 * lane + 3] = isNew ? cm.q[a] : -cm.q[a];
+      }
+      waveSyncLds();
+      de = waveEwaldDelta(S, 2 * cm.na, W.at, W.tab, lane, true);
+    }
+    const double dU = Unew - Uold + de;
+    if (uAcc < exp(-R.beta * dU))
+    {
+      if (lane < cm.na)
+      {
+        const int a0 = m * MC_MAXATOMS + lane;
+        S.ax[a0] = W.pos[3 * (cm.na + lane)];
+        S.ay[a0] = W.pos[3 * (cm.na + lane) + 1];
+        S.az[a0] = W.pos[3 * (cm.na + lane) + 2];
+      }
+      if (ew) waveAcceptEwald(S, lane);
+      L.U += dU;
+      L.accepted[isTrans ? 2 : 3] += 1;
+    }
+    return;
+  }
+  const double fugVol = R.beta * c.fugacity * S.box.volume;
+  if (uMove < pRot + 0.5 * (1.0 - pRot))
+  {
+    // CBMC insertion (src/raspakit/mc_moves/component/insertion_cbmc.cpp:34-200)
+    L.trials[0] += 1;
+    if (L.nmol >= R.maxMolecules) return;
+    double logW, Uext;
+    if (!growRigid(R, S, Y, c, sys, L.nads, step, -1, lane, W, logW, Uext)) return;
+    const double de = ewaldInsertRemove(R, S, L.netCharge, c, +1, lane, W, true);
+    const int nc = L.ncomp[ci];
+    // Pacc = exp(-beta dE_Ewald) beta f V / (N + 1) W / W_IG, W_IG = 1 (insertion_cbmc.cpp:157-168)
+    const double logPacc = -R.beta * de + log(fugVol / double(nc + 1)) + logW;
+    if (log(uAcc) < logPacc)
+    {
+      const int m = L.nmol;
+      if (lane < MC_MAXATOMS)
+      {
+        const int a0 = m * MC_MAXATOMS + lane;
+        const bool real = lane < c.na;
+        S.ax[a0] = real ? W.mol[3 * lane] : 0.0;
+        S.ay[a0] = real ? W.mol[3 * lane + 1] : 0.0;
+        S.az[a0] = real ? W.mol[3 * lane + 2] : 0.0;
+        S.aq[a0] = real ? c.q[lane] : 0.0;
+        S.at[a0] = real ? c.type[lane] : -1;
+        S.amol[a0] = m;
+      }
+      if (lane == 0) Y.comp[m] = ci;
+      if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
+      L.nmol = m + 1;
+      L.ncomp[ci] = nc + 1;
+      L.nads = (m + 1) * MC_MAXATOMS;
+      L.netCharge += c.qsum;
+      L.U += Uext + de;
+      L.accepted[0] += 1;
+    }
+    return;
+  }
+  // CBMC deletion (src/raspakit/mc_moves/component/deletion_cbmc.cpp:67-174)
+  L.trials[1] += 1;
+  const int nc = L.ncomp[ci];
+  if (nc == 0) return;
+  const int kth = min(int(uMol * nc), nc - 1);
+  int m = -1;
+  {
+    int seen = 0;
+    for (int base = 0; base < L.nmol && m < 0; base += 64)
+    {
+      const int q = base + lane;
+      const bool is = q < L.nmol && Y.comp[q] == ci;
+      const unsigned long long b = __ballot(is);
+      const int cnt = __popcll(b);
+      if (kth < seen + cnt)
+      {
+        unsigned long long bb = b;
+        for (int r = 0; r < kth - seen; ++r) bb &= bb - 1;
+        m = base + __ffsll((long long)bb) - 1;
+      }
+      seen += cnt;
+    }
+  }
+  if (m < 0) return;
+  double logW, Uext;
+  if (!growRigid(R, S, Y, c, sys, L.nads, step, m, lane, W, logW, Uext)) return;
+  const double de = ewaldInsertRemove(R, S, L.netCharge, c, -1, lane, W, true);
+  // Pacc = exp(-beta dE_Ewald) N / (beta f V) W_IG / W (deletion_cbmc.cpp:159-161)
+  const double logPacc = -R.beta * de + log(double(nc) / fugVol) - logW;
+  if (log(uAcc) < logPacc)
+  {
+    if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
+    // move the last molecule into slot m
+    const int last = L.nmol - 1;
+    if (m != last)
+    {
+      if (lane < MC_MAXATOMS)
+      {
+        const int d = m * MC_MAXATOMS + lane, s = last * MC_MAXATOMS + lane;
+        S.ax[d] = S.ax[s];
+        S.ay[d] = S.ay[s];
+        S.az[d] = S.az[s];
+        S.aq[d] = S.aq[s];
+        S.at[d] = S.at[s];
+      }
+      if (lane == 0) Y.comp[m] = Y.comp[last];
+    }
+    L.nmol = last;
+    L.ncomp[ci] = nc - 1;
+    L.nads = last * MC_MAXATOMS;
+    L.netCharge -= c.qsum;
+    L.U += -Uext + de;
+    L.accepted[1] += 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
+                                              McSys* __restrict__ Yarr, int nsys, int cycles, int sample)
+{
+  __shared__ WaveScratch scratch[4];
+  const McRun& R = *Rp;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sys = blockIdx.x * 4 + w;
+  if (sys >= nsys) return;
+  SysDev& S = Sarr[sys];
+  McSys& Y = Yarr[sys];
+  Live L;
+  L.nmol = Y.nmol;
+  for (int q = 0; q < MC_MAXCOMP; ++q) L.ncomp[q] = Y.ncomp[q];
+  L.nads = S.nads;
+  L.U = Y.U;
+  L.netCharge = S.netCharge;
+  L.step = Y.step;
+  L.cycle = Y.cycle;
+  for (int q = 0; q < 4; ++q)
+  {
+    L.trials[q] = Y.stat.trials[q];
+    L.accepted[q] = Y.stat.accepted[q];
+  }
+  for (int cyc = 0; cyc < cycles; ++cyc)
+  {
+    // steps per cycle = max(20, N) (src/raspakit/drivers/monte_carlo.cpp:301)
+    const int nsteps = max(R.minStepsPerCycle, L.nmol);
+    for (int s = 0; s < nsteps; ++s)
+    {
+      gcmcStep(R, S, Y, sys, L, lane, scratch[w]);
+      waveSyncGlobal();
+      L.step += 1;
+    }
+    if (sample)
+    {
+      // block sums live in global memory, read and written by lane 0 only
+      if (lane == 0)
+      {
+        const int b = min(MC_BLOCKS - 1, int(L.cycle * MC_BLOCKS / max(1LL, R.productionCycles)));
+        McBlock& B = Y.block[b];
+        B.count += 1.0;
+        B.U += L.U;
+        for (int q = 0; q < R.ncomp; ++q)
+        {
+          const double n = L.ncomp[q];
+          B.N[q] += n;
+          B.UN[q] += L.U * n;
+          for (int r = 0; r < R.ncomp; ++r) B.NN[q][r] += n * L.ncomp[r];
+        }
+      }
+      L.cycle += 1;
+    }
+  }
+  if (lane == 0)
+  {
+    Y.nmol = L.nmol;
+    for (int q = 0; q < MC_MAXCOMP; ++q) Y.ncomp[q] = L.ncomp[q];
+    S.nads = L.nads;
+    Y.U = L.U;
+    S.netCharge = L.netCharge;
+    Y.step = L.step;
+    Y.cycle = L.cycle;
+    for (int q = 0; q < 4; ++q)
+    {
+      Y.stat.trials[q] = L.trials[q];
+      Y.stat.accepted[q] = L.accepted[q];
+    }
+  }
+}
+
+// Recomputes the adsorbate energy of every system from scratch (framework-molecule + molecule-molecule pairs, and
+// the Ewald Fourier energy from the stored structure factor with the self and exclusion terms): the drift check of
+// the reference's integration tests (tests/raspa3-tests/mc_muvt_grids_drift.cpp:93-115) on the device.
+__global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ Rp, const SysDev* __restrict__ Sarr,
+                                                      const McSys* __restrict__ Yarr, int nsys, double* out)
+{
+  const McRun& R = *Rp;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int sys = blockIdx.x * 4 + w;
+  if (sys >= nsys) return;
+  const SysDev& S = Sarr[sys];
+  const McSys& Y = Yarr[sys];
+  double e = 0.0;
+  for (int m = 0; m < Y.nmol; ++m)
+  {
+    const Comp& c = R.comp[Y.comp[m]];
+    for (int a = 0; a < c.na; ++a)
+    {
+      const int a0 = m * MC_MAXATOMS + a;
+      bool o;
+      // pairs with molecules of lower index only, so each inter-molecular pair counts once
+      double ef = siteExternal(S, 0, S.ax[a0], S.ay[a0], S.az[a0], c.type[a], c.q[a], m, -2, lane, o);
+      double ei = 0.0;
+      for (int b = lane; b < m * MC_MAXATOMS; b += 64)
+      {
+        const int t = S.at[b];
+        if (t < 0) continue;
+        double dx = S.ax[b] - S.ax[a0], dy = S.ay[b] - S.ay[a0], dz = S.az[b] - S.az[a0];
+        rgd::pbc(S.box, dx, dy, dz);
+        const double rr = dx * dx + dy * dy + dz * dz;
+        if (rr < S.rcv2Mol) ei += rgd::vdw0(S.pairs[t * S.ntypes + c.type[a]], 1.0, rr);
+        if (S.useCharge && rr < S.rcc2) ei += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.aq[b] * c.q[a]);
+      }
+      e += ef + rgd::waveSum(ei);
+    }
+  }
+  // Ewald Fourier energy of the adsorbates from the stored structure factor
+  double ef = 0.0;
+  if (S.useCharge)
+  {
+    for (int k = lane; k < S.nk; k += 64)
+    {
+      const double2 s = S.Sk[k], f = S.Sfw[k];
+      ef += S.fk[k] * ((s.x * s.x + s.y * s.y) - (f.x * f.x + f.y * f.y));
+    }
+    ef = rgd::waveSum(ef);
+    double netAds = 0.0;
+    for (int m = 0; m < Y.nmol; ++m)
+    {
+      const Comp& c = R.comp[Y.comp[m]];
+      ef -= c.eIntra;
+      netAds += c.qsum;
+    }
+    const double fwNet = S.netCharge - netAds;
+    const double uIon = -(S.singleIon - S.coulomb * S.alpha / sqrt(M_PI));
+    ef += uIon * (2.0 * fwNet + netAds) * netAds;
+  }
+  if (lane == 0) out[sys] = e + ef;
+}
+
+}  // namespace rg

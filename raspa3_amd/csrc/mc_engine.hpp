@@ -1,0 +1,89 @@
+// Monte Carlo engine data shared between the device kernels (mc_engine.hip) and the host API (mc_engine_api.hip).
+#pragma once
+
+#include "mc_internal.hpp"
+
+#define MC_MAXATOMS 4   // atoms per rigid molecule (slots per molecule in the adsorbate arrays)
+#define MC_MAXCOMP 2    // components per system
+#define MC_BLOCKS 5     // block averages (src/raspakit/properties/property_block_average.ixx)
+#define MC_MAXTRIALS 64 // first-bead / orientation trials (one lane each)
+
+namespace rg
+{
+
+// A rigid component: body-frame atom positions relative to the first bead, types, charges, and the constant
+// Ewald self + intra-molecular exclusion energy of one molecule.
+struct Comp
+{
+  int na;
+  int first;
+  int charged;
+  int pad;
+  double ref[MC_MAXATOMS][3];
+  int type[MC_MAXATOMS];
+  double q[MC_MAXATOMS];
+  double qsum;
+  double eIntra;      // C alpha/sqrt(pi) sum q^2 + sum_pairs C qi qj erf(alpha r)/r  (internal units)
+  double molFraction;
+  double fugacity;    // y phi P in internal units
+};
+
+// Tabulated first-bead proposal: probabilities p_v over nvox voxels of the cell, alias table, and
+// log(nvox p_v) (the log of V q on voxel v).
+struct Proposal
+{
+  int nvox;
+  int n[3];
+  const double* aliasProb;
+  const int* alias;
+  const double* logNvoxP;
+};
+
+struct McBlock
+{
+  double count;
+  double U;
+  double N[MC_MAXCOMP];
+  double UN[MC_MAXCOMP];
+  double NN[MC_MAXCOMP][MC_MAXCOMP];
+};
+
+struct McStat
+{
+  long long trials[4];    // insertion, deletion, translation, rotation
+  long long accepted[4];
+};
+
+// Per-system mutable Monte Carlo state (device).
+struct McSys
+{
+  int nmol;
+  int ncomp[MC_MAXCOMP];
+  int pad;
+  double U;               // running adsorbate energy (framework-molecule + molecule-molecule + Ewald), internal units
+  long long step;         // move counter = Philox stream position
+  long long cycle;        // production cycles sampled
+  int* comp;              // component of each molecule slot
+  Proposal prop;
+  McStat stat;
+  McBlock block[MC_BLOCKS];
+};
+
+// Run parameters shared by all systems of a launch (kernel argument).
+struct McRun
+{
+  unsigned long long seed;
+  double beta;
+  double logMinW;         // log(minimumRosenbluthFactor)
+  int kFirst, kOrient;
+  int ncomp;
+  int maxMolecules;
+  int minStepsPerCycle;
+  int pad;
+  double pTranslation, pRotation;  // swap probability = 1 - both, insertion/deletion 50/50
+  double maxTranslation, maxRotation;
+  long long productionCycles;
+  Comp comp[MC_MAXCOMP];
+};
+
+}  // namespace rg

@@ -1,0 +1,422 @@
+// Host side of the Monte Carlo engine: multi-system GCMC state, proposal tables, hipGraph-replayed cycles, Widom.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mc_engine.hpp"
+
+namespace rg
+{
+__global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample);
+__global__ void k_widom(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first, int count,
+                        double* outLogW, double* outU, double* outEwald);
+__global__ void k_total_energy(const McRun* Rp, const SysDev* Sarr, const McSys* Yarr, int nsys, double* out);
+}  // namespace rg
+
+using namespace rg;
+
+struct rg_mc
+{
+  rg_ctx* ctx = nullptr;
+  int device = 0;
+  std::vector<rg_system*> systems;
+  McRun run{};
+  McRun* dRun = nullptr;  // device copy read by the kernels
+  SysDev* dS = nullptr;  // contiguous copies of the systems' device views (the engine's working state)
+  McSys* dY = nullptr;
+  std::vector<McSys> hY;  // host mirror of the static parts (pointers)
+  std::vector<void*> owned;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  int graphSample = -1;
+};
+
+namespace
+{
+
+rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_component_desc* comps,
+                   const rg_mc_params* p, McRun& R)
+{
+  if (ncomp < 1 || ncomp > MC_MAXCOMP) return fail(ctx, RG_ERR_UNSUPPORTED, "1..%d components", MC_MAXCOMP);
+  const Units& u = units();
+  std::memset(&R, 0, sizeof(R));
+  R.seed = p->seed;
+  R.beta = 1.0 / (u.kB * p->temperature);
+  R.logMinW = std::log(1e-150);  // minimumRosenbluthFactor (src/raspakit/forcefield/forcefield.ixx:143)
+  R.kFirst = p->k_first;
+  R.kOrient = p->k_orient;
+  if (R.kFirst < 1 || R.kFirst > MC_MAXTRIALS || R.kOrient < 1 || R.kOrient > MC_MAXTRIALS)
+    return fail(ctx, RG_ERR_INVALID, "trial counts must lie in [1, %d]", MC_MAXTRIALS);
+  R.ncomp = ncomp;
+  R.maxMolecules = p->max_molecules;
+  R.minStepsPerCycle = std::max(1, p->min_steps_per_cycle);
+  R.pTranslation = p->p_translation;
+  R.pRotation = p->p_rotation;
+  R.maxTranslation = p->max_translation;
+  R.maxRotation = p->max_rotation;
+  R.productionCycles = std::max(1LL, p->production_cycles);
+  // pressure in internal units: Pa / (mass unit / (length unit * time unit^2)) (src/raspakit/units.cpp:54-56)
+  const double pressureConversion = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12);
+  double ysum = 0.0;
+  for (int c = 0; c < ncomp; ++c) ysum += comps[c].mol_fraction;
+  if (!(ysum > 0.0)) return fail(ctx, RG_ERR_INVALID, "mole fractions must sum to a positive value");
+  for (int c = 0; c < ncomp; ++c)
+  {
+    const rg_component_desc& d = comps[c];
+    Comp& C = R.comp[c];
+    if (d.n_atoms < 1 || d.n_atoms > MC_MAXATOMS || d.first_bead < 0 || d.first_bead >= d.n_atoms)
+      return fail(ctx, RG_ERR_UNSUPPORTED, "component %d: 1..%d atoms", c, MC_MAXATOMS);
+    C.na = d.n_atoms;
+    C.first = d.first_bead;
+    C.qsum = 0.0;
+    for (int a = 0; a < d.n_atoms; ++a)
+    {
+      if (d.type[a] < 0 || d.type[a] >= ff->n) return fail(ctx, RG_ERR_INVALID, "component %d type out of range", c);
+      for (int k = 0; k < 3; ++k) C.ref[a][k] = d.position[3 * a + k] - d.position[3 * d.first_bead + k];
+      C.type[a] = d.type[a];
+      C.q[a] = d.charge[a];
+      C.qsum += d.charge[a];
+    }
+    C.charged = 0;
+    for (int a = 0; a < d.n_atoms; ++a) C.charged |= d.charge[a] != 0.0;
+    // Ewald self and intra-molecular exclusion of one rigid molecule (src/raspakit/interactions/ewald.cpp:1165-1234)
+    double self = 0.0, excl = 0.0;
+    const double ps = u.coulomb * ff->alpha / std::sqrt(M_PI);
+    for (int a = 0; a < d.n_atoms; ++a)
+    {
+      self += ps * d.charge[a] * d.charge[a];
+      for (int b = a + 1; b < d.n_atoms; ++b)
+      {
+        double r2 = 0.0;
+        for (int k = 0; k < 3; ++k) r2 += (C.ref[a][k] - C.ref[b][k]) * (C.ref[a][k] - C.ref[b][k]);
+        const double r = std::sqrt(r2);
+        if (r > 0.0) excl += u.coulomb * d.charge[a] * d.charge[b] * std::erf(ff->alpha * r) / r;
+      }
+    }
+    C.eIntra = ff->use_charge ? self + excl : 0.0;
+    C.molFraction = d.mol_fraction / ysum;
+    C.fugacity = C.molFraction * p->pressure_pa / pressureConversion;  // fugacity coefficient 1
+  }
+  return RG_OK;
+}
+
+}  // namespace
+
+extern "C"
+{
+
+rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, int n_components,
+                       const rg_component_desc* components, const rg_mc_params* params, rg_mc** out)
+{
+  if (!ctx || n_systems <= 0 || !systems || !components || !params || !out)
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_create: invalid arguments");
+  const rg_forcefield* ff = systems[0]->ff;
+  for (int s = 0; s < n_systems; ++s)
+  {
+    if (!systems[s]) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: system %d is null", s);
+    if (systems[s]->ff != ff) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: systems must share one force field");
+    if (systems[s]->h.nads != 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: systems must start empty");
+    if (systems[s]->h.cap < params->max_molecules * MC_MAXATOMS)
+      return fail(ctx, RG_ERR_INVALID, "rg_mc_create: system %d capacity below %d atoms", s,
+                  params->max_molecules * MC_MAXATOMS);
+  }
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  rg_mc* mc = new rg_mc();
+  mc->ctx = ctx;
+  mc->device = ctx->device;
+  mc->systems.assign(systems, systems + n_systems);
+  rg_status st = buildRun(ctx, ff, n_components, components, params, mc->run);
+  if (st != RG_OK)
+  {
+    delete mc;
+    return st;
+  }
+  auto alloc = [&](void** p, size_t bytes) -> rg_status
+  {
+    RG_HIP(ctx, hipMalloc(p, bytes));
+    mc->owned.push_back(*p);
+    return RG_OK;
+  };
+  std::vector<SysDev> hS(n_systems);
+  mc->hY.assign(n_systems, McSys{});
+  for (int s = 0; s < n_systems; ++s)
+  {
+    hS[s] = systems[s]->h;
+    McSys& Y = mc->hY[s];
+    std::memset(&Y, 0, sizeof(Y));
+    void* p = nullptr;
+    if ((st = alloc(&p, sizeof(int) * std::max(1, params->max_molecules))) != RG_OK) break;
+    Y.comp = static_cast<int*>(p);
+  }
+  if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dRun), sizeof(McRun));
+  if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dS), sizeof(SysDev) * n_systems);
+  if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dY), sizeof(McSys) * n_systems);
+  if (st != RG_OK)
+  {
+    rg_mc_destroy(mc);
+    return st;
+  }
+  RG_HIP(ctx, hipMemcpyAsync(mc->dRun, &mc->run, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(mc->dS, hS.data(), sizeof(SysDev) * n_systems, hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(mc->dY, mc->hY.data(), sizeof(McSys) * n_systems, hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *out = mc;
+  return RG_OK;
+}
+
+rg_status rg_mc_destroy(rg_mc* mc)
+{
+  if (!mc) return RG_ERR_INVALID;
+  (void)hipSetDevice(mc->device);
+  if (mc->ctx) (void)hipStreamSynchronize(mc->ctx->stream);
+  // hand the adsorbate count and net charge back to the systems
+  if (mc->dS && !mc->systems.empty())
+  {
+    std::vector<SysDev> hS(mc->systems.size());
+    if (hipMemcpy(hS.data(), mc->dS, sizeof(SysDev) * hS.size(), hipMemcpyDeviceToHost) == hipSuccess)
+      for (size_t s = 0; s < hS.size(); ++s)
+      {
+        mc->systems[s]->h.nads = hS[s].nads;
+        mc->systems[s]->h.netCharge = hS[s].netCharge;
+        (void)hipMemcpy(mc->systems[s]->d, &mc->systems[s]->h, sizeof(SysDev), hipMemcpyHostToDevice);
+      }
+  }
+  if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
+  if (mc->graph) (void)hipGraphDestroy(mc->graph);
+  for (void* p : mc->owned) (void)hipFree(p);
+  delete mc;
+  return RG_OK;
+}
+
+rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability)
+{
+  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || !n3)
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: invalid arguments");
+  McSys& Y = mc->hY[system];
+  if (!probability)
+  {
+    Y.prop = Proposal{};
+  }
+  else
+  {
+    const int nvox = n3[0] * n3[1] * n3[2];
+    if (nvox <= 0 || n3[0] <= 0 || n3[1] <= 0 || n3[2] <= 0)
+      return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: bad voxel grid");
+    double sum = 0.0;
+    for (int v = 0; v < nvox; ++v)
+    {
+      if (!(probability[v] > 0.0)) return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: probabilities must be > 0");
+      sum += probability[v];
+    }
+    // Walker / Vose alias table of the normalised probabilities
+    std::vector<double> prob(nvox), scaled(nvox), logp(nvox);
+    std::vector<int> alias(nvox, 0);
+    for (int v = 0; v < nvox; ++v)
+    {
+      const double pv = probability[v] / sum;
+      scaled[v] = pv * nvox;
+      logp[v] = std::log(pv * nvox);
+    }
+    std::vector<int> small, large;
+    for (int v = 0; v < nvox; ++v) (scaled[v] < 1.0 ? small : large).push_back(v);
+    while (!small.empty() && !large.empty())
+    {
+      const int s = small.back(), l = large.back();
+      small.pop_back();
+      prob[s] = scaled[s];
+      alias[s] = l;
+      scaled[l] = (scaled[l] + scaled[s]) - 1.0;
+      if (scaled[l] < 1.0)
+      {
+        large.pop_back();
+        small.push_back(l);
+      }
+    }
+    for (int v : large) prob[v] = 1.0, alias[v] = v;
+    for (int v : small) prob[v] = 1.0, alias[v] = v;
+    void *dp, *da, *dl;
+    RG_HIP(ctx, hipMalloc(&dp, nvox * sizeof(double)));
+    mc->owned.push_back(dp);
+    RG_HIP(ctx, hipMalloc(&da, nvox * sizeof(int)));
+    mc->owned.push_back(da);
+    RG_HIP(ctx, hipMalloc(&dl, nvox * sizeof(double)));
+    mc->owned.push_back(dl);
+    RG_HIP(ctx, hipMemcpyAsync(dp, prob.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(ctx, hipMemcpyAsync(da, alias.data(), nvox * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(ctx, hipMemcpyAsync(dl, logp.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    Y.prop.nvox = nvox;
+    for (int a = 0; a < 3; ++a) Y.prop.n[a] = n3[a];
+    Y.prop.aliasProb = static_cast<const double*>(dp);
+    Y.prop.alias = static_cast<const int*>(da);
+    Y.prop.logNvoxP = static_cast<const double*>(dl);
+  }
+  // only the proposal descriptor changes: copy it into the device state
+  RG_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char*>(mc->dY + system) + offsetof(McSys, prop), &Y.prop,
+                             sizeof(Proposal), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph)
+{
+  if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  const int nsys = static_cast<int>(mc->systems.size());
+  const dim3 grid((nsys + 3) / 4), block(256);
+  if (!use_graph)
+  {
+    for (int c = 0; c < cycles; ++c)
+    {
+      hipLaunchKernelGGL(k_gcmc, grid, block, 0, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+      RG_HIP(ctx, hipGetLastError());
+    }
+  }
+  else
+  {
+    // one MC cycle of every system captured once as a graph, replayed per cycle
+    if (!mc->graphExec || mc->graphSample != sample)
+    {
+      if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
+      if (mc->graph) (void)hipGraphDestroy(mc->graph);
+      mc->graphExec = nullptr;
+      mc->graph = nullptr;
+      RG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_gcmc, grid, block, 0, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+      hipError_t e = hipStreamEndCapture(ctx->stream, &mc->graph);
+      if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
+      RG_HIP(ctx, hipGraphInstantiate(&mc->graphExec, mc->graph, nullptr, nullptr, 0));
+      mc->graphSample = sample;
+    }
+    for (int c = 0; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
+  }
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
+rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out)
+{
+  if (!ctx || !mc || !out || system < 0 || system >= int(mc->systems.size()))
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_state: invalid arguments");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  McSys Y;
+  RG_HIP(ctx, hipMemcpyAsync(&Y, mc->dY + system, sizeof(McSys), hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->n_molecules = Y.nmol;
+  for (int c = 0; c < MC_MAXCOMP; ++c) out->n_per_component[c] = Y.ncomp[c];
+  out->energy = Y.U;
+  out->steps = Y.step;
+  out->cycles_sampled = Y.cycle;
+  for (int q = 0; q < 4; ++q)
+  {
+    out->trials[q] = Y.stat.trials[q];
+    out->accepted[q] = Y.stat.accepted[q];
+  }
+  for (int b = 0; b < MC_BLOCKS; ++b)
+  {
+    out->block_count[b] = Y.block[b].count;
+    out->block_energy[b] = Y.block[b].U;
+    for (int c = 0; c < MC_MAXCOMP; ++c)
+    {
+      out->block_n[b][c] = Y.block[b].N[c];
+      out->block_energy_n[b][c] = Y.block[b].UN[c];
+      for (int d = 0; d < MC_MAXCOMP; ++d) out->block_nn[b][c][d] = Y.block[b].NN[c][d];
+    }
+  }
+  return RG_OK;
+}
+
+rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacity, double* pos, int* type,
+                           int* component, int* n_out)
+{
+  if (!ctx || !mc || !n_out || system < 0 || system >= int(mc->systems.size()))
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_adsorbates: invalid arguments");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  McSys Y;
+  SysDev S;
+  RG_HIP(ctx, hipMemcpyAsync(&Y, mc->dY + system, sizeof(McSys), hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(&S, mc->dS + system, sizeof(SysDev), hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int n = Y.nmol * MC_MAXATOMS;
+  *n_out = n;
+  if (n > capacity || n == 0) return n > capacity ? fail(ctx, RG_ERR_INVALID, "capacity below %d", n) : RG_OK;
+  std::vector<double> x(n), y(n), z(n);
+  std::vector<int> t(n), c(Y.nmol);
+  RG_HIP(ctx, hipMemcpyAsync(x.data(), S.ax, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(y.data(), S.ay, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(z.data(), S.az, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(t.data(), S.at, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(c.data(), Y.comp, Y.nmol * 4, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int a = 0; a < n; ++a)
+  {
+    if (pos)
+    {
+      pos[3 * a] = x[a];
+      pos[3 * a + 1] = y[a];
+      pos[3 * a + 2] = z[a];
+    }
+    if (type) type[a] = t[a];
+  }
+  if (component)
+    for (int m = 0; m < Y.nmol; ++m) component[m] = c[m];
+  return RG_OK;
+}
+
+rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out)
+{
+  if (!ctx || !mc || !out) return fail(ctx, RG_ERR_INVALID, "rg_mc_total_energy: invalid arguments");
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  const int nsys = static_cast<int>(mc->systems.size());
+  double* d = nullptr;
+  RG_HIP(ctx, hipMalloc(&d, nsys * sizeof(double)));
+  hipLaunchKernelGGL(k_total_energy, dim3((nsys + 3) / 4), dim3(256), 0, ctx->stream, mc->dRun, mc->dS, mc->dY,
+                     nsys, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, nsys * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "rg_mc_total_energy");
+  return RG_OK;
+}
+
+rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component, const rg_mc_params* params,
+                   long long first, int count, double* log_weight, double* energy, double* ewald)
+{
+  if (!ctx || !sys || !component || !params || count < 0 || (count > 0 && (!log_weight || !energy || !ewald)))
+    return fail(ctx, RG_ERR_INVALID, "rg_widom: invalid arguments");
+  if (count == 0) return RG_OK;
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  McRun R;
+  rg_status st = buildRun(ctx, sys->ff, 1, component, params, R);
+  if (st != RG_OK) return st;
+  void* d = nullptr;
+  const size_t runBytes = (sizeof(McRun) + 255) / 256 * 256, sysBytes = (sizeof(McSys) + 255) / 256 * 256;
+  RG_HIP(ctx, hipMalloc(&d, runBytes + sysBytes + 3 * size_t(count) * sizeof(double)));
+  McSys Y;
+  std::memset(&Y, 0, sizeof(Y));
+  McRun* dR = static_cast<McRun*>(d);
+  McSys* dY = reinterpret_cast<McSys*>(static_cast<char*>(d) + runBytes);
+  double* out = reinterpret_cast<double*>(static_cast<char*>(d) + runBytes + sysBytes);
+  hipError_t e = hipMemcpyAsync(dR, &R, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dY, &Y, sizeof(McSys), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+  {
+    hipLaunchKernelGGL(k_widom, dim3((count + 3) / 4), dim3(256), 0, ctx->stream, dR, sys->d, dY, 0, first, count,
+                       out, out + count, out + 2 * size_t(count));
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(log_weight, out, count * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(energy, out + count, count * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ewald, out + 2 * size_t(count), count * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "rg_widom");
+  return RG_OK;
+}
+
+}  // extern "C"

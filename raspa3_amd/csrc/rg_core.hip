@@ -174,6 +174,17 @@ rg_status rg_box(double a, double b, double c, double alpha_deg, double beta_deg
   return RG_OK;
 }
 
+rg_status rg_ewald_parameters(double precision, double rc_coulomb, const double* widths3, double* alpha, int* kmax3)
+{
+  if (!widths3 || !alpha || !kmax3 || !(rc_coulomb > 0.0)) return RG_ERR_INVALID;
+  const double eps = std::min(std::fabs(precision), 0.5);
+  const double tol = std::sqrt(std::abs(std::log(eps * rc_coulomb)));
+  *alpha = std::sqrt(std::abs(std::log(eps * rc_coulomb * tol))) / rc_coulomb;
+  const double tol1 = std::sqrt(-std::log(eps * rc_coulomb * (2.0 * tol * *alpha) * (2.0 * tol * *alpha)));
+  for (int a = 0; a < 3; ++a) kmax3[a] = static_cast<int>(std::rint(0.25 + widths3[a] * *alpha * tol1 / M_PI));
+  return RG_OK;
+}
+
 rg_status rg_grid_dims(const double* cell9, double spacing, int* n3)
 {
   if (!cell9 || !n3 || !(spacing > 0.0)) return RG_ERR_INVALID;

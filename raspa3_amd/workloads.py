@@ -192,6 +192,50 @@ def framework_config1():
         "config1_framework", H, rect, (13, 15, 7), 2.0, 1000, 1001, 0, 1, 2.05, -1.025))
 
 
+def framework_small_charged():
+    """A 25 A cubic charged silicate for fast Monte Carlo tests: 7 x 7 x 7 Si + 686 O = 1029 sites, neutral."""
+    H, rect = cell_orthorhombic(25.0, 25.0, 25.0)
+    return _cached("small_charged_framework", lambda: generate_silicate(
+        "small_charged_framework", H, rect, (7, 7, 7), 2.2, 2000, 2001, 0, 1, 2.05, -1.025))
+
+
+def co2_component(spec, mol_fraction=1.0):
+    """Rigid CO2 of configuration 1 (C 27 K 2.80 A +0.70; O 79 K 3.05 A -0.35; C-O 1.16 A); first bead = C."""
+    return dict(n_atoms=3, first_bead=1, position=[[0, 0, 1.16], [0, 0, 0], [0, 0, -1.16]],
+                type=[spec.index("O_co2"), spec.index("C_co2"), spec.index("O_co2")], charge=[-0.35, 0.70, -0.35],
+                mol_fraction=mol_fraction)
+
+
+def n2_component(spec, mol_fraction=1.0):
+    """N2 of configuration 4: two LJ sites (36 K, 3.31 A, -0.482 e) 0.55 A either side of a +0.964 e centre;
+    first bead = the centre."""
+    return dict(n_atoms=3, first_bead=1, position=[[0, 0, 0.55], [0, 0, 0], [0, 0, -0.55]],
+                type=[spec.index("N_n2"), spec.index("N_com"), spec.index("N_n2")], charge=[-0.482, 0.964, -0.482],
+                mol_fraction=mol_fraction)
+
+
+def forcefield_co2_n2():
+    """Configuration-1 silicate + CO2 + N2 (configuration 4)."""
+    return _ff([("Si", 0.0, 1.0, 2.05, True), ("O", 44.0, 3.3, -1.025, True),
+                ("C_co2", 27.0, 2.80, 0.70, False), ("O_co2", 79.0, 3.05, -0.35, False),
+                ("N_n2", 36.0, 3.31, -0.482, False), ("N_com", 0.0, 1.0, 0.964, False)],
+               alpha=0.265, use_charge=True)
+
+
+def methane_component(spec):
+    return dict(n_atoms=1, first_bead=0, position=[[0, 0, 0]], type=[spec.index("CH4")], charge=[0.0],
+                mol_fraction=1.0)
+
+
+def mc_params(temperature=300.0, pressure_pa=1e5, seed=42, k_first=10, k_orient=10, p_translation=0.25,
+              p_rotation=0.25, max_translation=0.5, max_rotation=0.5, min_steps_per_cycle=20, max_molecules=256,
+              production_cycles=1000):
+    return dict(temperature=temperature, pressure_pa=pressure_pa, k_first=k_first, k_orient=k_orient,
+                p_translation=p_translation, p_rotation=p_rotation, max_translation=max_translation,
+                max_rotation=max_rotation, min_steps_per_cycle=min_steps_per_cycle, max_molecules=max_molecules,
+                seed=seed, production_cycles=production_cycles)
+
+
 def grid_dims(cell, spacing):
     """n = (size_t)(w / h + 0.5) from perpendicular widths (src/raspakit/system_interpolation_grids.cpp:222-226)."""
     a, b, c = cell[:, 0], cell[:, 1], cell[:, 2]
