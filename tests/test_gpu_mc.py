@@ -1,0 +1,178 @@
+"""GPU Monte Carlo engine against the oracle's replay of the same moves (through the C ABI).
+
+Both draw every trial from the same Philox4x32-10 stream positions, so a Widom insertion or a GCMC trajectory is
+replayed move by move: the selected trials and every accept/reject decision must agree, and energies must agree to
+FP64 summation-order differences.  Tolerances: 1e-9 relative on energies and log Rosenbluth weights, 1e-8 A on
+adsorbate positions after a run; move counters exactly.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import raspa3_amd as ra
+from raspa3_amd import workloads as w
+from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def close(a, b, rel=1e-9, abs_=1e-9):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.all((np.abs(a - b) <= abs_ + rel * np.abs(b)) | (np.isneginf(a) & np.isneginf(b)))
+
+
+def test_widom_methane_itq29_grid(gpu_ctx):
+    """Config-0 style Widom (single-site CH4, grid path) in the reference's ITQ-29 fixture, 3x3x3 box."""
+    print("case: widom CH4 ITQ-29", flush=True)
+    d = json.load(open(os.path.join(GOLDEN, "itq29.json")))
+    rows = json.load(open(os.path.join(GOLDEN, "reference_values.json")))["zeolite_forcefield"]["pseudo_atoms"]
+    spec = w._ff([(r[0], r[2], r[3], 0.0, r[1]) for r in rows], use_charge=False)
+    a = d["a"]
+    frac = np.array(d["fractional"])
+    typ = np.array([spec.index(e) for e in d["element"]], np.int32)
+    unit = np.diag([a, a, a])
+    ff = ra.ForceField(gpu_ctx, spec)
+    fu = ra.Framework(gpu_ctx, w.Framework("itq29", unit, True, frac * a, typ, np.zeros(len(typ))))
+    dims = ra.grid_dims(unit, 0.3)
+    g = ra.Grid(gpu_ctx, unit, True, dims)
+    ra.build_grids(gpu_ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, spec.index("CH4"), 12.0)])
+    big = np.diag([3 * a] * 3)
+    pos = np.vstack([frac * a + a * np.array([i, j, k]) for i in range(3) for j in range(3) for k in range(3)])
+    fwb = w.Framework("itq29x3", big, True, pos, np.tile(typ, 27), np.zeros(len(pos)))
+    fb = ra.Framework(gpu_ctx, fwb)
+    s = ra.System(gpu_ctx, ff, big, True, framework=fb, grids={spec.index("CH4"): g}, kmax=(0, 0, 0), capacity=64)
+    comp = w.methane_component(spec)
+    params = w.mc_params(seed=1234)
+    lw, u, ew = ra.widom(gpu_ctx, s, comp, params, 0, 4096)
+    grid = g.download().reshape(dims[2], dims[1], dims[0], 8)
+    om = oracle.MC(spec, fwb, [comp], params, kmax=(0, 0, 0), grids=({spec.index("CH4"): grid}, None, dims, unit))
+    olw, ou, oew = om.widom(0, 0, 4096)
+    assert close(lw, olw) and close(u, ou)
+    W = np.exp(lw).mean()
+    assert W > 1.0  # methane adsorbs in the LTA cages
+
+
+def test_widom_co2_config1_direct_ewald(gpu_ctx):
+    """CO2 Widom in the 4095-site charged triclinic framework: direct framework sums, orientation trials and the
+    Ewald Fourier difference at kmax = 8."""
+    print("case: widom CO2 config1", flush=True)
+    fw, spec = w.framework_config1(), w.forcefield_config1()
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, kmax=(8, 8, 8), capacity=64)
+    comp = w.co2_component(spec)
+    params = w.mc_params(seed=99)
+    lw, u, ew = ra.widom(gpu_ctx, s, comp, params, 100, 32)
+    om = oracle.MC(spec, fw, [comp], params, kmax=(8, 8, 8))
+    olw, ou, oew = om.widom(0, 100, 32)
+    assert close(lw, olw) and close(u, ou) and close(ew, oew, abs_=1e-7)
+    assert np.isfinite(lw).sum() > 0
+
+
+def small_charged_setup(ctx, spec, types, spacing=0.3):
+    fw = w.framework_small_charged()
+    ff = ra.ForceField(ctx, spec)
+    f = ra.Framework(ctx, fw)
+    dims = ra.grid_dims(fw.cell, spacing)
+    grids = {t: ra.Grid(ctx, fw.cell, fw.rect, dims) for t in types}
+    coul = ra.Grid(ctx, fw.cell, fw.rect, dims)
+    ra.build_grids(ctx, ff, f, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
+                   [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    host = {t: g.download().reshape(dims[2], dims[1], dims[0], 8) for t, g in grids.items()}
+    hcoul = coul.download().reshape(dims[2], dims[1], dims[0], 8)
+    return fw, ff, f, dims, grids, coul, host, hcoul
+
+
+def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, dims, cycles, proposal=None):
+    om = oracle.MC(spec, fw, comps, params, kmax=kmax, system=sysidx, grids=(host, hcoul, dims))
+    if proposal is not None:
+        om.set_proposal(*proposal)
+    om.run(cycles[0])
+    om.run(cycles[1], sample=True)
+    o = om.state()
+    g = mc.state(sysidx)
+    assert g["n_molecules"] == o["n_molecules"]
+    assert list(g["trials"]) == list(o["trials"]) and list(g["accepted"]) == list(o["accepted"])
+    assert close(g["energy"], o["energy"], rel=1e-8, abs_=1e-6)
+    gp, gt, gc = mc.adsorbates(sysidx)
+    op, ot, oc = om.adsorbates()
+    assert np.array_equal(gt, ot) and np.array_equal(gc, oc)
+    assert np.allclose(gp, op, rtol=0, atol=1e-8)
+    return g
+
+
+def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx):
+    """Two independent CO2 systems (grid path, Ewald kmax 6) for 5 + 15 cycles at 1e6 Pa."""
+    print("case: gcmc CO2 replay", flush=True)
+    spec = w.forcefield_config1()
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types)
+    comp = w.co2_component(spec)
+    params = w.mc_params(pressure_pa=1e6, seed=5, max_molecules=128, production_cycles=15)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                         capacity=4 * 128) for _ in range(2)]
+    mc = ra.Mc(gpu_ctx, systems, [comp], params)
+    mc.run(5, sample=False, graph=False)
+    mc.run(15, sample=True, graph=True)
+    for sidx in range(2):
+        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (6, 6, 6), host, hcoul, dims, (5, 15))
+        assert g["accepted"][0] > 0
+    # drift: running energy equals the energy recomputed from scratch
+    tot = mc.total_energy()
+    for sidx in range(2):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
+
+
+def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
+    """CO2/N2 80:20 with a tabulated first-bead proposal (random positive voxel probabilities)."""
+    print("case: gcmc mixture + proposal replay", flush=True)
+    spec = w.forcefield_co2_n2()
+    types = [spec.index(n) for n in ("C_co2", "O_co2", "N_n2", "N_com")]
+    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types)
+    comps = [w.co2_component(spec, 0.8), w.n2_component(spec, 0.2)]
+    params = w.mc_params(pressure_pa=2e6, seed=77, max_molecules=128, production_cycles=10)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                  capacity=4 * 128)
+    mc = ra.Mc(gpu_ctx, [s], comps, params)
+    rng = np.random.default_rng(3)
+    n3 = [8, 8, 8]
+    prob = rng.uniform(0.2, 1.0, 512)
+    mc.set_proposal(0, n3, prob)
+    mc.run(4, sample=False, graph=True)
+    mc.run(10, sample=True, graph=True)
+    compare_with_oracle(mc, 0, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, (4, 10), proposal=(n3, prob))
+    st = mc.state(0)
+    assert st["n_per_component"][0] > 0
+    mc.close()
+
+
+def test_graph_replay_equals_eager(gpu_ctx):
+    """Replaying one captured MC cycle gives the same bits as launching it eagerly, with host synchronisation and
+    state queries between the replays."""
+    print("case: graph vs eager", flush=True)
+    spec = w.forcefield_config1()
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types, spacing=0.4)
+    comp = w.co2_component(spec)
+    params = w.mc_params(pressure_pa=1e6, seed=8, max_molecules=64, production_cycles=6)
+    runs = []
+    for graph in (False, True):
+        systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul,
+                             kmax=(6, 6, 6), capacity=4 * 64) for _ in range(3)]
+        mc = ra.Mc(gpu_ctx, systems, [comp], params)
+        trace = []
+        for _ in range(6):
+            mc.run(1, sample=True, graph=graph)
+            trace.append([mc.state(i)["energy"] for i in range(3)])
+        runs.append((trace, [mc.adsorbates(i)[0] for i in range(3)]))
+        mc.close()
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert np.array_equal(a, b)
