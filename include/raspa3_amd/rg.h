@@ -262,6 +262,39 @@ rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacit
 /* Adsorbate energy of every system recomputed from scratch (System::computeTotalEnergies, src/raspakit/system.cpp:690). */
 rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out);
 
+/* ---------------------------------------------------------------------------------------------
+ * Learned first-bead proposal.  The reference has no counterpart: its first-bead trials are uniform in the cell
+ * (src/raspakit/cbmc/multiple_first_bead.cpp:26-27 for insertion, :56-75 for retrace).  An MLP on periodic Fourier
+ * features of fractional coordinates, 192 -> 256 -> 256 -> 256 -> 1 with SiLU, gives an unnormalised log-density
+ * f(s); BF16 weights and activations, FP32 accumulation (MFMA), FP32 master weights updated by Adam.
+ * Trained by minibatch softmax cross-entropy against Boltzmann weights exp(-beta U_i).  rg_mlp_proposal tabulates
+ * it at voxel centres with a uniform floor, the exactly normalised density handed to rg_mc_set_proposal.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct rg_mlp rg_mlp;
+
+/* Number of FP32 parameters: W1 (192 x 256), b1, W2 (256 x 256), b2, W3 (256 x 256), b3, w4 (256), b4, in this
+ * order, W[in][out] row-major; input q = 64 a + 2 (m - 1) + c is sin (c = 0) or cos (c = 1) of 2 pi m s_a. */
+size_t rg_mlp_num_params(void);
+/* He-uniform weights from a splitmix64 stream of 'seed', zero biases. */
+rg_status rg_mlp_create(rg_ctx* ctx, unsigned long long seed, rg_mlp** out);
+rg_status rg_mlp_destroy(rg_mlp* mlp);
+rg_status rg_mlp_get_params(rg_ctx* ctx, const rg_mlp* mlp, float* out);
+rg_status rg_mlp_set_params(rg_ctx* ctx, rg_mlp* mlp, const float* in);
+/* f(s) at n points (fractional coordinates, 3 per point). */
+rg_status rg_mlp_forward(rg_ctx* ctx, rg_mlp* mlp, int n, const double* frac, double* logdensity);
+/* Loss -sum_i p_i log softmax(f)_i with p = softmax(-beta U) over one batch (n a multiple of 64, <= 65536) and its
+ * gradient with respect to every parameter; the parameters are not changed. */
+rg_status rg_mlp_gradients(rg_ctx* ctx, rg_mlp* mlp, int n, const double* frac, const double* energy, double beta,
+                           float* grads, double* loss);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over 'epochs' shuffled passes of minibatches of 'batch' points (a multiple
+ * of 64, <= 65536 and <= n; the n % batch points left over in an epoch are skipped); epoch_loss[epochs] (may be
+ * NULL) receives the mean minibatch loss of every epoch. */
+rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* mlp, int n, const double* frac, const double* energy, double beta,
+                       int batch, int epochs, double lr, unsigned long long seed, double* epoch_loss);
+/* probability[v] = (1 - floor) exp(f(c_v)) / sum_w exp(f(c_w)) + floor / nvox at the centres c_v of
+ * n3[0] x n3[1] x n3[2] voxels (x fastest); 0 < floor <= 1. */
+rg_status rg_mlp_proposal(rg_ctx* ctx, rg_mlp* mlp, const int* n3, double floor_fraction, double* probability);
+
 #ifdef __cplusplus
 }
 #endif

@@ -346,3 +346,65 @@ class Mc:
             self.close()
         except Exception:
             pass
+
+
+class Mlp:
+    """rg_mlp: the learned first-bead proposal network (BF16 MFMA training and inference on the device)."""
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        L.check(lib().rg_mlp_create(ctx.h, C.c_ulonglong(seed), C.byref(h)), ctx.h, "rg_mlp_create")
+        self.h = h
+        self.n_params = int(lib().rg_mlp_num_params())
+
+    def params(self):
+        out = np.zeros(self.n_params, np.float32)
+        L.check(lib().rg_mlp_get_params(self.ctx.h, self.h, L.f32p(out)), self.ctx.h, "rg_mlp_get_params")
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32).reshape(-1)
+        assert w.size == self.n_params
+        L.check(lib().rg_mlp_set_params(self.ctx.h, self.h, L.f32p(w)), self.ctx.h, "rg_mlp_set_params")
+
+    def forward(self, frac):
+        frac = L.f64(frac).reshape(-1, 3)
+        out = np.zeros(len(frac))
+        L.check(lib().rg_mlp_forward(self.ctx.h, self.h, len(frac), L.d(frac), L.d(out)), self.ctx.h,
+                "rg_mlp_forward")
+        return out
+
+    def gradients(self, frac, energy, beta):
+        frac = L.f64(frac).reshape(-1, 3)
+        energy = L.f64(energy).reshape(-1)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        L.check(lib().rg_mlp_gradients(self.ctx.h, self.h, len(frac), L.d(frac), L.d(energy), beta, L.f32p(g),
+                                       C.byref(loss)), self.ctx.h, "rg_mlp_gradients")
+        return g, loss.value
+
+    def train(self, frac, energy, beta, batch=4096, epochs=1, lr=1e-3, seed=1):
+        frac = L.f64(frac).reshape(-1, 3)
+        energy = L.f64(energy).reshape(-1)
+        losses = np.zeros(max(epochs, 1))
+        L.check(lib().rg_mlp_train(self.ctx.h, self.h, len(frac), L.d(frac), L.d(energy), beta, batch, epochs, lr,
+                                   C.c_ulonglong(seed), L.d(losses)), self.ctx.h, "rg_mlp_train")
+        return losses[:epochs]
+
+    def proposal(self, n3, floor=0.05):
+        n = L.i32(n3)
+        out = np.zeros(int(np.prod(n3)))
+        L.check(lib().rg_mlp_proposal(self.ctx.h, self.h, L.i(n), floor, L.d(out)), self.ctx.h, "rg_mlp_proposal")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().rg_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -133,3 +133,22 @@ SIGNATURES.update({
     "rg_mc_adsorbates": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, ip, ip, ip]),
     "rg_mc_total_energy": (C.c_int, [vp, vp, dp]),
 })
+
+fp = C.POINTER(C.c_float)
+
+SIGNATURES.update({
+    "rg_mlp_num_params": (C.c_size_t, []),
+    "rg_mlp_create": (C.c_int, [vp, C.c_ulonglong, C.POINTER(vp)]),
+    "rg_mlp_destroy": (C.c_int, [vp]),
+    "rg_mlp_get_params": (C.c_int, [vp, vp, fp]),
+    "rg_mlp_set_params": (C.c_int, [vp, vp, fp]),
+    "rg_mlp_forward": (C.c_int, [vp, vp, C.c_int, dp, dp]),
+    "rg_mlp_gradients": (C.c_int, [vp, vp, C.c_int, dp, dp, C.c_double, fp, dp]),
+    "rg_mlp_train": (C.c_int, [vp, vp, C.c_int, dp, dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_ulonglong,
+                               dp]),
+    "rg_mlp_proposal": (C.c_int, [vp, vp, ip, C.c_double, dp]),
+})
+
+
+def f32p(a):
+    return a.ctypes.data_as(fp) if a is not None else None
