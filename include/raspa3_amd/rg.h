@@ -259,6 +259,15 @@ rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* o
 /* Adsorbate atoms of a system (4 slots per molecule; unused slots have type -1) and the component of each molecule. */
 rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacity, double* pos, int* type,
                            int* component, int* n_out);
+/* Training data of the learned proposal: the orientation-averaged Boltzmann factor of a rigid molecule with its
+ * first bead at each of n points (fractional coordinates of the system's box), framework energy from the system's
+ * interpolation grids (every atom type of the component needs one): log_weight[i] = log (1/M) sum_o exp(-U(s_i, R_o)
+ * / kT) over M = n_orient (a power of two <= 64) uniformly random orientations drawn from the Philox stream
+ * (point i, orientation o) of 'seed'; -inf where every orientation overlaps.  No reference counterpart: it is the
+ * first-bead marginal of the orientation trials (src/raspakit/cbmc/operators.cpp:958-968). */
+rg_status rg_molecule_weights(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,
+                              double temperature, int n, const double* frac, int n_orient, unsigned long long seed,
+                              double* log_weight);
 /* Adsorbate energy of every system recomputed from scratch (System::computeTotalEnergies, src/raspakit/system.cpp:690). */
 rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out);
 

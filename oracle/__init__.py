@@ -299,6 +299,8 @@ def _mc_lib():
         L.orc_mc_adsorbates.argtypes = [vp, dp, ip, ip]
         L.orc_mc_adsorbates.restype = C.c_int
         L.orc_widom.argtypes = [vp, C.c_int, C.c_longlong, C.c_int, dp, dp, dp]
+        L.orc_mc_molecule_weights.restype = C.c_int
+        L.orc_mc_molecule_weights.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_ulonglong, dp]
         L._mc_ready = True
     return L
 
@@ -386,6 +388,13 @@ class MC:
         lw, u, ew = np.zeros(count), np.zeros(count), np.zeros(count)
         _mc_lib().orc_widom(self.h, component, first, count, _d(lw), _d(u), _d(ew))
         return lw, u, ew
+
+    def molecule_weights(self, component, frac, n_orient, seed):
+        frac = _f(frac).reshape(-1, 3)
+        out = np.zeros(len(frac))
+        st = _mc_lib().orc_mc_molecule_weights(self.h, component, len(frac), _d(frac), n_orient, seed, _d(out))
+        assert st == 0, "every atom type of the component needs a grid"
+        return out
 
     def __del__(self):
         try:

@@ -289,6 +289,17 @@ def widom(ctx, system, comp, params, first, count):
     return lw, u, ew
 
 
+def molecule_weights(ctx, system, comp, temperature, frac, n_orient=16, seed=1):
+    """log of the orientation-averaged Boltzmann factor of a rigid molecule with its first bead at each point
+    (fractional coordinates of the system's box), framework energy from the grids."""
+    frac = L.f64(frac).reshape(-1, 3)
+    out = np.empty(len(frac))
+    desc = component_desc(comp)
+    L.check(lib().rg_molecule_weights(ctx.h, system.h, C.byref(desc), temperature, len(frac), L.d(frac), n_orient,
+                                      C.c_ulonglong(seed), L.d(out)), ctx.h, "rg_molecule_weights")
+    return out
+
+
 class Mc:
     """rg_mc: GCMC over several systems (one wavefront each)."""
 

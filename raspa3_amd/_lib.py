@@ -132,6 +132,8 @@ SIGNATURES.update({
     "rg_mc_state": (C.c_int, [vp, vp, C.c_int, C.POINTER(McState)]),
     "rg_mc_adsorbates": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, ip, ip, ip]),
     "rg_mc_total_energy": (C.c_int, [vp, vp, dp]),
+    "rg_molecule_weights": (C.c_int, [vp, vp, C.POINTER(ComponentDesc), C.c_double, C.c_int, dp, C.c_int,
+                                      C.c_ulonglong, dp]),
 })
 
 fp = C.POINTER(C.c_float)

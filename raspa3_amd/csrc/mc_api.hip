@@ -184,6 +184,7 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
       return bail(st);
     h.gridOfType = dg;
     h.grids = dv;
+    s->gridOfType = got;
   }
   // adsorbates
   h.cap = capacity_atoms;

@@ -485,6 +485,50 @@ struct Live
   long long accepted[4];
 };
 
+// Orientation-averaged Boltzmann factor of a rigid molecule with its first bead at each of n points (fractional
+// coordinates of the box), from the interpolation grids only: log W(s) = log (1/M) sum_o exp(-beta U(s, R_o)) over M
+// random orientations (Philox stream (point, orientation) of 'seed'); -inf when every orientation overlaps.  W(s) is
+// the first-bead density that the orientation trials of growRigid reward, the training target of the learned
+// proposal.  M is a power of two up to 64; a group of M lanes takes one point.
+__global__ __launch_bounds__(256) void k_molecule_weights(const SysDev* __restrict__ Sp, Comp c, double beta, int n,
+                                                          const double* __restrict__ frac, int M, unsigned long long seed,
+                                                          double* __restrict__ logW)
+{
+  const SysDev& S = *Sp;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / M, o = t % M;
+  const bool active = i < n;
+  double e = 0.0;
+  bool ov = false;
+  if (active)
+  {
+    const double f[3] = {frac[3 * size_t(i)], frac[3 * size_t(i) + 1], frac[3 * size_t(i) + 2]};
+    double p[3];
+    for (int k = 0; k < 3; ++k) p[k] = S.box.cell[k] * f[0] + S.box.cell[3 + k] * f[1] + S.box.cell[6 + k] * f[2];
+    double u1, u2, u3, u4, Rm[9];
+    rgd::uniform2(seed, uint32_t(i), uint32_t(o), 0u, 0u, u1, u2);
+    rgd::uniform2(seed, uint32_t(i), uint32_t(o), 0u, 1u, u3, u4);
+    randomRotation(u1, u2, u3, Rm);
+    for (int a = 0; a < c.na; ++a)
+    {
+      const double* r = c.ref[a];
+      const double x = p[0] + Rm[0] * r[0] + Rm[1] * r[1] + Rm[2] * r[2];
+      const double y = p[1] + Rm[3] * r[0] + Rm[4] * r[1] + Rm[5] * r[2];
+      const double z = p[2] + Rm[6] * r[0] + Rm[7] * r[1] + Rm[8] * r[2];
+      const double g = rgd::interpolate(S.grids[S.gridOfType[c.type[a]]], x, y, z);
+      ov = ov || g > S.overlap;
+      e += g;
+      if (S.useCharge && S.coulGrid >= 0) e += c.q[a] * rgd::interpolate(S.grids[S.coulGrid], x, y, z);
+    }
+  }
+  const double a = (active && !ov) ? -beta * e : -INFINITY;
+  double m = a;
+  for (int off = 1; off < M; off <<= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  double sum = (m == -INFINITY) ? 0.0 : exp(a - m);
+  for (int off = 1; off < M; off <<= 1) sum += __shfl_xor(sum, off, 64);
+  if (active && o == 0) logW[i] = (m == -INFINITY) ? -INFINITY : m + log(sum / M);
+}
+
 __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W)
 {
   const unsigned long long step = (unsigned long long)L.step;

@@ -12,6 +12,8 @@ __global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int
 __global__ void k_widom(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first, int count,
                         double* outLogW, double* outU, double* outEwald);
 __global__ void k_total_energy(const McRun* Rp, const SysDev* Sarr, const McSys* Yarr, int nsys, double* out);
+__global__ void k_molecule_weights(const SysDev* Sp, Comp c, double beta, int n, const double* frac, int M,
+                                   unsigned long long seed, double* logW);
 }  // namespace rg
 
 using namespace rg;
@@ -416,6 +418,49 @@ rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* c
   (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return hipfail(ctx, e, "rg_widom");
+  return RG_OK;
+}
+
+rg_status rg_molecule_weights(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,
+                              double temperature, int n, const double* frac, int n_orient, unsigned long long seed,
+                              double* log_weight)
+{
+  if (!ctx || !sys || !component || n < 0 || (n > 0 && (!frac || !log_weight)) || !(temperature > 0.0) ||
+      n_orient < 1 || n_orient > 64 || (n_orient & (n_orient - 1)))
+    return fail(ctx, RG_ERR_INVALID, "rg_molecule_weights: invalid arguments (n_orient: a power of two up to 64)");
+  if (n == 0) return RG_OK;
+  rg_mc_params params;
+  std::memset(&params, 0, sizeof(params));
+  params.temperature = temperature;
+  params.k_first = params.k_orient = 1;
+  McRun R;
+  rg_status st = buildRun(ctx, sys->ff, 1, component, &params, R);
+  if (st != RG_OK) return st;
+  for (int a = 0; a < component->n_atoms; ++a)
+  {
+    const int t = component->type[a];
+    if (t >= int(sys->gridOfType.size()) || sys->gridOfType[t] < 0)
+      return fail(ctx, RG_ERR_UNSUPPORTED, "rg_molecule_weights: atom %d (type %d) has no interpolation grid", a, t);
+  }
+  RG_HIP(ctx, hipSetDevice(ctx->device));
+  void* d = nullptr;
+  RG_HIP(ctx, hipMalloc(&d, size_t(n) * 4 * sizeof(double)));
+  double* dfrac = static_cast<double*>(d);
+  double* dout = dfrac + 3 * size_t(n);
+  hipError_t e = hipMemcpyAsync(dfrac, frac, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+  {
+    const size_t threads = size_t(n) * n_orient;
+    hipLaunchKernelGGL(k_molecule_weights, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, ctx->stream, sys->d,
+                       R.comp[0], R.beta, n, dfrac, n_orient, seed, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(log_weight, dout, size_t(n) * sizeof(double), hipMemcpyDeviceToHost,
+                                          ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hipfail(ctx, e, "rg_molecule_weights");
   return RG_OK;
 }
 

@@ -54,6 +54,7 @@ struct rg_system
   rg::SysDev* d = nullptr;     // device copy
   std::vector<void*> owned;    // device buffers freed on destroy
   std::vector<rgd::GridView> gridViews;
+  std::vector<int> gridOfType;  // host copy of h.gridOfType (-1: direct sums for that type)
   double fwNetCharge = 0.0;
   double pendingCharge = 0.0;  // charge change of the last rg_ewald_delta, applied by rg_ewald_accept
   long long intCut2 = 0;

@@ -176,3 +176,21 @@ def test_graph_replay_equals_eager(gpu_ctx):
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
+
+
+def test_molecule_weights_match_oracle(gpu_ctx):
+    """Training data of the learned proposal: orientation-averaged Boltzmann factors of CO2 from the grids."""
+    print("case: molecule weights", flush=True)
+    spec = w.forcefield_config1()
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types, spacing=0.4)
+    comp = w.co2_component(spec)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                  capacity=64)
+    frac = np.random.default_rng(2).random((777, 3))
+    for m in (1, 16, 64):
+        lw = ra.molecule_weights(gpu_ctx, s, comp, 300.0, frac, n_orient=m, seed=17)
+        om = oracle.MC(spec, fw, [comp], w.mc_params(), kmax=(6, 6, 6), grids=(host, hcoul, dims))
+        olw = om.molecule_weights(0, frac, m, 17)
+        assert close(lw, olw, rel=1e-9, abs_=1e-9), m
+    assert np.isfinite(lw).any() and np.isneginf(lw).any()

@@ -66,3 +66,26 @@ def test_gcmc_low_pressure_follows_henry_law():
     # Widom standard error ~2.5 %, block error ~1.5 %, finite-loading correction ~2 % at this loading
     assert abs(mean / target - 1.0) < 0.1, mean
     assert s["accepted"][0] > 1000 and s["accepted"][1] > 1000
+
+
+def test_gcmc_biased_first_bead_proposal_keeps_henry_law():
+    """The same Henry's-law loading with a deliberately non-uniform first-bead proposal (random voxel probabilities
+    over 8^3 voxels): the weight exp(-beta U) / (V q) in growth and retrace keeps the acceptance rule exact, so a
+    missing or inverted 1/(V q) factor moves <N> away from the target."""
+    spec, fw, grids = itq29_methane_setup()
+    comp = w.methane_component(spec)
+    beta = 1.0 / (oracle.units()["kB"] * 300.0)
+    vol = abs(np.linalg.det(fw.cell))
+    pconv = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+    wid = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=1.0, seed=7), grids=grids)
+    W = np.exp(wid.widom(0, 0, 20000)[0]).mean()
+    target = 0.25
+    P = target / (beta * vol * W) * pconv
+    mc = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=P, p_translation=0.5, p_rotation=0.0,
+                                                 production_cycles=8000, seed=21), grids=grids)
+    mc.set_proposal([8, 8, 8], np.random.default_rng(4).uniform(0.05, 1.0, 512))
+    mc.run(100)
+    mc.run(8000, sample=True)
+    b = mc.state()["blocks"]
+    mean = (b[:, 2] / b[:, 0]).mean()
+    assert abs(mean / target - 1.0) < 0.1, mean
