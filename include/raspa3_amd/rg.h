@@ -42,6 +42,7 @@ typedef struct rg_grid rg_grid;
 
 /* Creates a context on HIP device 'device' with its own stream. */
 rg_status rg_ctx_create(int device, rg_ctx** out);
+/* The *_destroy functions wait for the device and do not use the context, so objects may be released after it. */
 rg_status rg_ctx_destroy(rg_ctx* ctx);
 /* Last error message of this context (empty string if none).  Valid until the next call. */
 const char* rg_last_error(const rg_ctx* ctx);
@@ -248,12 +249,14 @@ rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* c
 /* GCMC over n_systems independent, initially empty systems sharing one force field; one wavefront per system. */
 rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, int n_components,
                        const rg_component_desc* components, const rg_mc_params* params, rg_mc** out);
-/* Hands the final adsorbate count and net charge back to the systems and releases the engine. */
+/* Releases the engine.  The systems hold the adsorbates, Ewald state and net charge of the last rg_mc_run, so they
+ * may be released before or after the engine (but not used by the engine once released). */
 rg_status rg_mc_destroy(rg_mc* mc);
 /* Learned first-bead proposal for one system: probability[v] > 0 over n3[0]*n3[1]*n3[2] voxels of the cell (x fastest);
  * NULL restores the uniform proposal. */
 rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability);
-/* Runs 'cycles' MC cycles; sample != 0 accumulates block averages; use_graph != 0 replays one captured cycle. */
+/* Runs 'cycles' MC cycles; sample != 0 accumulates block averages; use_graph != 0 replays one captured cycle.
+ * On return the systems' adsorbate arrays, structure factors, adsorbate count and net charge are the final state. */
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);
 rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
 /* Adsorbate atoms of a system (4 slots per molecule; unused slots have type -1) and the component of each molecule. */

@@ -6,6 +6,7 @@ in libraspa3_amd.so on the GPU; this module only moves host arrays in and out.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -36,7 +37,8 @@ def grid_dims(cell, spacing):
 
 
 class Context:
-    """rg_ctx: one HIP device and stream."""
+    """rg_ctx: one HIP device and stream.  Closing it first closes every object still open on it (newest first),
+    so that nothing is released after its context."""
 
     def __init__(self, device=0):
         h = C.c_void_p()
@@ -44,9 +46,18 @@ class Context:
         if st != L.RG_OK:
             raise RGError(f"rg_ctx_create failed with status {st} (no usable HIP device?)")
         self.h = h
+        self._children = []
+
+    def _own(self, obj):
+        self._children.append(weakref.ref(obj))
 
     def close(self):
         if self.h:
+            for r in reversed(self._children):
+                obj = r()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             lib().rg_ctx_destroy(self.h)
             self.h = None
 
@@ -67,8 +78,33 @@ class Context:
         return lib().rg_last_error(self.h).decode()
 
 
-class ForceField:
+class _Handle:
+    """A library object owned by a Context; 'close' releases it once (also on garbage collection)."""
+
+    _destroy = None
+    h = None
+
+    def _register(self, ctx, h):
+        self.h = h
+        if ctx is not None:
+            ctx._own(self)
+
+    def close(self):
+        if self.h:
+            getattr(lib(), self._destroy)(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ForceField(_Handle):
     """rg_forcefield built from a workloads.ForceFieldSpec (or any object with the same fields)."""
+
+    _destroy = "rg_forcefield_destroy"
 
     def __init__(self, ctx, spec):
         self.ctx = ctx
@@ -79,7 +115,7 @@ class ForceField:
                                            int(spec.shifted), spec.rc_framework, spec.rc_molecule, spec.rc_coulomb,
                                            spec.alpha, int(spec.use_charge), spec.overlap, C.byref(h)),
                 ctx.h if ctx else None, "rg_forcefield_create")
-        self.h = h
+        self._register(ctx, h)
         self.n = len(eps)
 
     def pairs(self):
@@ -89,16 +125,12 @@ class ForceField:
         L.check(lib().rg_forcefield_pairs(self.h, L.d(eps), L.d(sig), L.d(sh), L.i(lj)), None, "pairs")
         return eps.reshape(n, n), sig.reshape(n, n), sh.reshape(n, n), lj.reshape(n, n)
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().rg_forcefield_destroy(self.h)
-        except Exception:
-            pass
 
 
-class Framework:
+class Framework(_Handle):
     """rg_framework from a workloads.Framework."""
+
+    _destroy = "rg_framework_destroy"
 
     def __init__(self, ctx, fw):
         self.ctx = ctx
@@ -108,18 +140,13 @@ class Framework:
         pos, typ, q = L.f64(fw.pos), L.i32(fw.type), L.f64(fw.charge)
         L.check(lib().rg_framework_create(ctx.h, L.d(c9), int(fw.rect), len(typ), L.d(pos), L.i(typ), L.d(q),
                                           C.byref(h)), ctx.h, "rg_framework_create")
-        self.h = h
-
-    def __del__(self):
-        try:
-            if self.h:
-                lib().rg_framework_destroy(self.h)
-        except Exception:
-            pass
+        self._register(ctx, h)
 
 
-class Grid:
+class Grid(_Handle):
     """rg_grid: a device-resident tricubic grid (8 values per point, layout_left)."""
+
+    _destroy = "rg_grid_destroy"
 
     def __init__(self, ctx, cell, rect, n):
         self.ctx = ctx
@@ -129,7 +156,7 @@ class Grid:
         h = C.c_void_p()
         c9 = L.f64(self.cell.T.reshape(9))
         L.check(lib().rg_grid_create(ctx.h, L.d(c9), int(rect), *self.n, C.byref(h)), ctx.h, "rg_grid_create")
-        self.h = h
+        self._register(ctx, h)
 
     @property
     def count(self):
@@ -153,12 +180,6 @@ class Grid:
                 "rg_grid_interpolate")
         return (e, g) if gradient else e
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().rg_grid_destroy(self.h)
-        except Exception:
-            pass
 
 
 def ewald_parameters(precision, rc, widths):
@@ -170,8 +191,10 @@ def ewald_parameters(precision, rc, widths):
     return a.value, [int(x) for x in k]
 
 
-class System:
+class System(_Handle):
     """rg_system: box, force field, framework (direct) and/or grids, adsorbates, Ewald state."""
+
+    _destroy = "rg_system_destroy"
 
     def __init__(self, ctx, ff, cell, rect, framework=None, grids=None, coulomb_grid=None, kmax=(8, 8, 8),
                  int_cut2=0, capacity=4096):
@@ -186,7 +209,7 @@ class System:
                                        len(grids), L.i(gtypes), garr, coulomb_grid.h if coulomb_grid else None,
                                        L.i(L.i32(kmax)), int(int_cut2), int(capacity), C.byref(h)),
                 ctx.h, "rg_system_create")
-        self.h = h
+        self._register(ctx, h)
 
     def info(self):
         nk, n = C.c_int(), C.c_int()
@@ -225,12 +248,6 @@ class System:
         L.check(lib().rg_ewald_energy(self.ctx.h, self.h, L.d(out)), self.ctx.h, "rg_ewald_energy")
         return out
 
-    def __del__(self):
-        try:
-            if self.h:
-                lib().rg_system_destroy(self.h)
-        except Exception:
-            pass
 
 
 def rosenbluth_select(ctx, energies, overlap, beta, uniforms):
@@ -300,8 +317,10 @@ def molecule_weights(ctx, system, comp, temperature, frac, n_orient=16, seed=1):
     return out
 
 
-class Mc:
+class Mc(_Handle):
     """rg_mc: GCMC over several systems (one wavefront each)."""
+
+    _destroy = "rg_mc_destroy"
 
     def __init__(self, ctx, systems, comps, params):
         self.ctx = ctx
@@ -312,7 +331,7 @@ class Mc:
         h = C.c_void_p()
         L.check(lib().rg_mc_create(ctx.h, len(systems), arr, len(comps), descs, C.byref(p), C.byref(h)), ctx.h,
                 "rg_mc_create")
-        self.h = h
+        self._register(ctx, h)
 
     def set_proposal(self, system, n3, prob):
         n = L.i32(n3)
@@ -347,26 +366,18 @@ class Mc:
         L.check(lib().rg_mc_total_energy(self.ctx.h, self.h, L.d(out)), self.ctx.h, "rg_mc_total_energy")
         return out
 
-    def close(self):
-        if self.h:
-            lib().rg_mc_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
-class Mlp:
+class Mlp(_Handle):
     """rg_mlp: the learned first-bead proposal network (BF16 MFMA training and inference on the device)."""
+
+    _destroy = "rg_mlp_destroy"
 
     def __init__(self, ctx, seed=1):
         self.ctx = ctx
         h = C.c_void_p()
         L.check(lib().rg_mlp_create(ctx.h, C.c_ulonglong(seed), C.byref(h)), ctx.h, "rg_mlp_create")
-        self.h = h
+        self._register(ctx, h)
         self.n_params = int(lib().rg_mlp_num_params())
 
     def params(self):
@@ -409,13 +420,3 @@ class Mlp:
         L.check(lib().rg_mlp_proposal(self.ctx.h, self.h, L.i(n), floor, L.d(out)), self.ctx.h, "rg_mlp_proposal")
         return out
 
-    def close(self):
-        if self.h:
-            lib().rg_mlp_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass

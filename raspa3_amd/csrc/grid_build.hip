@@ -550,7 +550,7 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
       return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_build: more than %d framework pseudo-atom types", MAXT);
     for (int t = 0; t < ff->n; ++t) typeRow[t] = seen[t] < 0 ? 0 : seen[t];
   }
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   const int* n = g0->n;
   const double* H = fw->cell;
   const double rc = ff->rc_framework;
@@ -756,7 +756,7 @@ rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size
 {
   if (!ctx || !grid || !host || offset + count > grid->count)
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_download: range out of bounds");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   RG_HIP(ctx, hipMemcpyAsync(host, grid->d + offset, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RG_OK;
@@ -766,7 +766,7 @@ rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count
 {
   if (!ctx || !grid || !host || offset + count > grid->count)
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_upload: range out of bounds");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   RG_HIP(ctx, hipMemcpyAsync(grid->d + offset, host, count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RG_OK;

@@ -139,7 +139,7 @@ extern "C" rg_status rg_grid_interpolate(rg_ctx* ctx, const rg_grid* grid, int n
   if (!ctx || !grid || n < 0 || (n > 0 && (!pos || !energy)))
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_interpolate: invalid arguments");
   if (n == 0) return RG_OK;
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   double* d = nullptr;
   const size_t bytes = size_t(n) * (3 + 1 + (gradient ? 3 : 0)) * sizeof(double);
   RG_HIP(ctx, hipMalloc(&d, bytes));

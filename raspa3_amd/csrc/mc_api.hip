@@ -83,7 +83,7 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
   for (int a = 0; a < 3; ++a)
     if (kmax3[a] < 0 || kmax3[a] > EW_MAXK)
       return fail(ctx, RG_ERR_UNSUPPORTED, "rg_system_create: kmax must lie in [0, %d]", EW_MAXK);
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   rg_system* s = new rg_system();
   s->ctx = ctx;
   s->device = ctx->device;
@@ -267,7 +267,8 @@ rg_status rg_system_destroy(rg_system* s)
 {
   if (!s) return RG_ERR_INVALID;
   (void)hipSetDevice(s->device);
-  if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+  // the context (and its stream) may already be gone: wait for the whole device instead
+  (void)hipDeviceSynchronize();
   for (void* p : s->owned) (void)hipFree(p);
   delete s;
   return RG_OK;
@@ -288,7 +289,7 @@ rg_status rg_system_set_adsorbates(rg_ctx* ctx, rg_system* s, int n, const doubl
 {
   if (!ctx || !s || n < 0 || n > s->h.cap || (n > 0 && (!pos || !type || !q || !molecule)))
     return fail(ctx, RG_ERR_INVALID, "rg_system_set_adsorbates: invalid arguments (capacity %d)", s ? s->h.cap : 0);
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   std::vector<double> x(n), y(n), z(n);
   double net = s->fwNetCharge;
   for (int a = 0; a < n; ++a)
@@ -333,7 +334,7 @@ rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* s, int n, const double
   if (n == 0) return RG_OK;
   for (int i = 0; i < n; ++i)
     if (type[i] < 0 || type[i] >= s->h.ntypes) return fail(ctx, RG_ERR_INVALID, "rg_trial_energies: type out of range");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   Scratch sc{ctx};
   const size_t bytes = size_t(n) * (3 * 8 + 4 + 8 + 4 + 4 * 8 + 4);
   RG_HIP(ctx, hipMalloc(&sc.p, bytes));
@@ -363,7 +364,7 @@ rg_status rg_rosenbluth_select(rg_ctx* ctx, int n_sets, int k, const double* ene
   if (!ctx || n_sets < 0 || k <= 0 || (n_sets > 0 && (!energies || !overlap || !uniforms || !selected || !log_weight)))
     return fail(ctx, RG_ERR_INVALID, "rg_rosenbluth_select: invalid arguments");
   if (n_sets == 0) return RG_OK;
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   Scratch sc{ctx};
   const size_t m = size_t(n_sets) * k;
   RG_HIP(ctx, hipMalloc(&sc.p, m * 12 + size_t(n_sets) * 20));
@@ -391,7 +392,7 @@ rg_status rg_ewald_delta(rg_ctx* ctx, rg_system* s, int n_new, const double* new
   if (!ctx || !s || !out || n_new < 0 || n_old < 0 || n_new + n_old > EW_MAXATOMS ||
       (n_new > 0 && (!new_pos || !new_q || !new_mol)) || (n_old > 0 && (!old_pos || !old_q || !old_mol)))
     return fail(ctx, RG_ERR_INVALID, "rg_ewald_delta: invalid arguments (at most %d atoms)", EW_MAXATOMS);
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   std::vector<double> at(4 * size_t(n_new + n_old));
   std::vector<int> mol(n_new + n_old);
   for (int a = 0; a < n_new; ++a)
@@ -427,7 +428,7 @@ rg_status rg_ewald_delta(rg_ctx* ctx, rg_system* s, int n_new, const double* new
 rg_status rg_ewald_accept(rg_ctx* ctx, rg_system* s)
 {
   if (!ctx || !s) return fail(ctx, RG_ERR_INVALID, "rg_ewald_accept: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   if (s->h.nk > 0)
     hipLaunchKernelGGL(k_copy_sk, dim3((s->h.nk + 255) / 256), dim3(256), 0, ctx->stream, s->h.Sk, s->h.St, s->h.nk);
   RG_HIP(ctx, hipGetLastError());
@@ -442,7 +443,7 @@ rg_status rg_ewald_accept(rg_ctx* ctx, rg_system* s)
 rg_status rg_ewald_energy(rg_ctx* ctx, const rg_system* s, double* out)
 {
   if (!ctx || !s || !out) return fail(ctx, RG_ERR_INVALID, "rg_ewald_energy: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   const SysDev& h = s->h;
   const int nb = 64;
   Scratch sc{ctx};

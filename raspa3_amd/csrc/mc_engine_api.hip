@@ -123,7 +123,7 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
       return fail(ctx, RG_ERR_INVALID, "rg_mc_create: system %d capacity below %d atoms", s,
                   params->max_molecules * MC_MAXATOMS);
   }
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   rg_mc* mc = new rg_mc();
   mc->ctx = ctx;
   mc->device = ctx->device;
@@ -171,19 +171,10 @@ rg_status rg_mc_destroy(rg_mc* mc)
 {
   if (!mc) return RG_ERR_INVALID;
   (void)hipSetDevice(mc->device);
-  if (mc->ctx) (void)hipStreamSynchronize(mc->ctx->stream);
-  // hand the adsorbate count and net charge back to the systems
-  if (mc->dS && !mc->systems.empty())
-  {
-    std::vector<SysDev> hS(mc->systems.size());
-    if (hipMemcpy(hS.data(), mc->dS, sizeof(SysDev) * hS.size(), hipMemcpyDeviceToHost) == hipSuccess)
-      for (size_t s = 0; s < hS.size(); ++s)
-      {
-        mc->systems[s]->h.nads = hS[s].nads;
-        mc->systems[s]->h.netCharge = hS[s].netCharge;
-        (void)hipMemcpy(mc->systems[s]->d, &mc->systems[s]->h, sizeof(SysDev), hipMemcpyHostToDevice);
-      }
-  }
+  // the context (and its stream) may already be gone: wait for the whole device instead
+  (void)hipDeviceSynchronize();
+  // the systems already hold the final adsorbate count and net charge (handed back after every run), so they may
+  // have been released before the engine
   if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
   if (mc->graph) (void)hipGraphDestroy(mc->graph);
   for (void* p : mc->owned) (void)hipFree(p);
@@ -263,7 +254,7 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph)
 {
   if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   const int nsys = static_cast<int>(mc->systems.size());
   const dim3 grid((nsys + 3) / 4), block(256);
   if (!use_graph)
@@ -292,6 +283,18 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
     }
     for (int c = 0; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
   }
+  // hand the adsorbate count and net charge back to the systems (their Ewald state and adsorbate arrays are
+  // updated in place by the kernel)
+  std::vector<SysDev> hS(nsys);
+  RG_HIP(ctx, hipMemcpyAsync(hS.data(), mc->dS, sizeof(SysDev) * nsys, hipMemcpyDeviceToHost, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < nsys; ++s)
+  {
+    rg_system* sy = mc->systems[s];
+    sy->h.nads = hS[s].nads;
+    sy->h.netCharge = hS[s].netCharge;
+    RG_HIP(ctx, hipMemcpyAsync(sy->d, &sy->h, sizeof(SysDev), hipMemcpyHostToDevice, ctx->stream));
+  }
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RG_OK;
 }
@@ -300,7 +303,7 @@ rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* o
 {
   if (!ctx || !mc || !out || system < 0 || system >= int(mc->systems.size()))
     return fail(ctx, RG_ERR_INVALID, "rg_mc_state: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   McSys Y;
   RG_HIP(ctx, hipMemcpyAsync(&Y, mc->dY + system, sizeof(McSys), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -334,7 +337,7 @@ rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacit
 {
   if (!ctx || !mc || !n_out || system < 0 || system >= int(mc->systems.size()))
     return fail(ctx, RG_ERR_INVALID, "rg_mc_adsorbates: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   McSys Y;
   SysDev S;
   RG_HIP(ctx, hipMemcpyAsync(&Y, mc->dY + system, sizeof(McSys), hipMemcpyDeviceToHost, ctx->stream));
@@ -369,7 +372,7 @@ rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacit
 rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out)
 {
   if (!ctx || !mc || !out) return fail(ctx, RG_ERR_INVALID, "rg_mc_total_energy: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   const int nsys = static_cast<int>(mc->systems.size());
   double* d = nullptr;
   RG_HIP(ctx, hipMalloc(&d, nsys * sizeof(double)));
@@ -390,7 +393,7 @@ rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* c
   if (!ctx || !sys || !component || !params || count < 0 || (count > 0 && (!log_weight || !energy || !ewald)))
     return fail(ctx, RG_ERR_INVALID, "rg_widom: invalid arguments");
   if (count == 0) return RG_OK;
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   McRun R;
   rg_status st = buildRun(ctx, sys->ff, 1, component, params, R);
   if (st != RG_OK) return st;
@@ -442,7 +445,7 @@ rg_status rg_molecule_weights(rg_ctx* ctx, const rg_system* sys, const rg_compon
     if (t >= int(sys->gridOfType.size()) || sys->gridOfType[t] < 0)
       return fail(ctx, RG_ERR_UNSUPPORTED, "rg_molecule_weights: atom %d (type %d) has no interpolation grid", a, t);
   }
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   void* d = nullptr;
   RG_HIP(ctx, hipMalloc(&d, size_t(n) * 4 * sizeof(double)));
   double* dfrac = static_cast<double*>(d);

@@ -443,7 +443,7 @@ size_t rg_mlp_num_params(void) { return NPARAM; }
 rg_status rg_mlp_create(rg_ctx* ctx, unsigned long long seed, rg_mlp** out)
 {
   if (!ctx || !out) return fail(ctx, RG_ERR_INVALID, "rg_mlp_create: invalid arguments");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   rg_mlp* p = new rg_mlp();
   p->ctx = ctx;
   p->device = ctx->device;
@@ -487,7 +487,8 @@ rg_status rg_mlp_destroy(rg_mlp* p)
 {
   if (!p) return RG_ERR_INVALID;
   (void)hipSetDevice(p->device);
-  if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+  // the context (and its stream) may already be gone: wait for the whole device instead
+  (void)hipDeviceSynchronize();
   freeBatch(p);
   void* bufs[] = {p->w, p->wb, p->g, p->m, p->v};
   for (void* b : bufs)
@@ -519,7 +520,7 @@ rg_status rg_mlp_forward(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, doub
   if (!ctx || !p || n < 0 || (n > 0 && (!frac || !logdensity)))
     return fail(ctx, RG_ERR_INVALID, "rg_mlp_forward: invalid arguments");
   if (n == 0) return RG_OK;
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   const int chunk = 1 << 16;
   rg_status st = ensureBatch(p, std::min(n, chunk));
   if (st != RG_OK) return st;
@@ -544,7 +545,7 @@ rg_status rg_mlp_gradients(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, co
 {
   if (!ctx || !p || n <= 0 || n % 64 != 0 || n > (1 << 16) || !frac || !energy || !grads || !loss)
     return fail(ctx, RG_ERR_INVALID, "rg_mlp_gradients: n must be a positive multiple of 64 up to 65536");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   rg_status st = ensureBatch(p, n);
   if (st != RG_OK) return st;
   RG_HIP(ctx, hipMemcpyAsync(p->frac, frac, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -563,7 +564,7 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
   if (!ctx || !p || n <= 0 || !frac || !energy || batch <= 0 || batch % 64 != 0 || batch > (1 << 16) ||
       batch > n || epochs < 0)
     return fail(ctx, RG_ERR_INVALID, "rg_mlp_train: invalid arguments (batch: multiple of 64, at most 65536 and n)");
-  RG_HIP(ctx, hipSetDevice(ctx->device));
+  RG_ENTER(ctx);
   rg_status st = ensureBatch(p, batch);
   if (st != RG_OK) return st;
   // the whole training set stays on the device; minibatches are gathered through a per-epoch permutation

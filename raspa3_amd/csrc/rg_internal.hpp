@@ -94,3 +94,13 @@ void widths3(const double* m, double* w);
     hipError_t rg_e_ = (call);                                 \
     if (rg_e_ != hipSuccess) return rg::hipfail(ctx, rg_e_, #call); \
   } while (0)
+
+// Entry of an API call: makes the context's device current and clears the error state that an earlier unchecked
+// HIP call (a release path, another library) may have left, so that the hipGetLastError after this call's own
+// launches reports this call's errors only.
+#define RG_ENTER(ctx)                           \
+  do                                            \
+  {                                             \
+    RG_HIP(ctx, hipSetDevice((ctx)->device));   \
+    (void)hipGetLastError();                    \
+  } while (0)

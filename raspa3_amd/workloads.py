@@ -222,6 +222,44 @@ def forcefield_co2_n2():
                alpha=0.265, use_charge=True)
 
 
+def forcefield_zeolite(extra=()):
+    """ForceField::makeZeoliteForceField (src/raspakit/forcefield/forcefield.cpp:1777-1841): Si 22 K 2.30 A +2.05,
+    O 53 K 3.30 A -1.025, CH4 158.5 K 3.72 A, C_co2 29.933 K 2.745 A +0.6512, O_co2 85.671 K 3.017 A -0.3256;
+    Ewald alpha from the automatic rule (ForceField::initializeEwaldParameters) for precision 1e-6 and 12 A.
+    'extra' appends (name, eps_K, sigma_A, charge) rows."""
+    rows = [("Si", 22.0, 2.30, 2.05, True), ("O", 53.0, 3.3, -1.025, True), ("CH4", 158.5, 3.72, 0.0, False),
+            ("C_co2", 29.933, 2.745, 0.6512, False), ("O_co2", 85.671, 3.017, -0.3256, False)]
+    rows += [(n, e, s_, q, False) for n, e, s_, q in extra]
+    eps, rc = 1e-6, 12.0
+    tol = math.sqrt(abs(math.log(eps * rc)))
+    alpha = math.sqrt(abs(math.log(eps * rc * tol))) / rc
+    return _ff(rows, alpha=alpha, use_charge=True)
+
+
+def framework_itq29(spec, rep=3):
+    """ITQ-29 (LTA, cubic a = 11.8671 A, 24 Si + 48 O per cell; the reference's test structure) repeated rep^3
+    times, with the force field's charges.  Returns (framework, unit cell)."""
+    with open(os.path.join(DATA_DIR, "itq29.json")) as f:
+        d = json.load(f)
+    a = d["a"]
+    frac = np.array(d["fractional"], float)
+    typ = np.array([spec.index(e) for e in d["element"]], np.int32)
+    shifts = np.array([[i, j, k] for i in range(rep) for j in range(rep) for k in range(rep)], float)
+    pos = ((frac[None, :, :] + shifts[:, None, :]) * a).reshape(-1, 3)
+    t = np.tile(typ, rep ** 3)
+    return (Framework(f"itq29x{rep}", np.diag([a * rep] * 3), True, pos, t, spec.charge[t].astype(float)),
+            np.diag([a, a, a]))
+
+
+def co2_reference(spec, mol_fraction=1.0):
+    """Component::makeCO2 (src/raspakit/component.cpp:2304-2329): O (0, 0, 1.149), C (0, 0, 0), O (0, 0, -1.149);
+    first bead = C."""
+    return dict(n_atoms=3, first_bead=1, position=[[0, 0, 1.149], [0, 0, 0], [0, 0, -1.149]],
+                type=[spec.index("O_co2"), spec.index("C_co2"), spec.index("O_co2")],
+                charge=[spec.charge[spec.index("O_co2")], spec.charge[spec.index("C_co2")],
+                        spec.charge[spec.index("O_co2")]], mol_fraction=mol_fraction)
+
+
 def methane_component(spec):
     return dict(n_atoms=1, first_bead=0, position=[[0, 0, 0]], type=[spec.index("CH4")], charge=[0.0],
                 mol_fraction=1.0)

@@ -88,8 +88,23 @@ def small_charged_setup(ctx, spec, types, spacing=0.3):
     return fw, ff, f, dims, grids, coul, host, hcoul
 
 
-def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, dims, cycles, proposal=None):
-    om = oracle.MC(spec, fw, comps, params, kmax=kmax, system=sysidx, grids=(host, hcoul, dims))
+def itq29_setup(ctx, spec, types, spacing=0.3):
+    """Charged ITQ-29, 3 x 3 x 3 cells (35.6 A), grids built on the unit cell."""
+    fw, unit = w.framework_itq29(spec, 3)
+    ff = ra.ForceField(ctx, spec)
+    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw.pos[:72], fw.type[:72], fw.charge[:72]))
+    dims = ra.grid_dims(unit, spacing)
+    grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
+    coul = ra.Grid(ctx, unit, True, dims)
+    ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
+                   [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    host = {t: g.download().reshape(dims[2], dims[1], dims[0], 8) for t, g in grids.items()}
+    hcoul = coul.download().reshape(dims[2], dims[1], dims[0], 8)
+    return fw, unit, ff, ra.Framework(ctx, fw), dims, grids, coul, host, hcoul
+
+
+def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, dims, gcell, cycles, proposal=None):
+    om = oracle.MC(spec, fw, comps, params, kmax=kmax, system=sysidx, grids=(host, hcoul, dims, gcell))
     if proposal is not None:
         om.set_proposal(*proposal)
     om.run(cycles[0])
@@ -107,21 +122,21 @@ def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, 
 
 
 def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx):
-    """Two independent CO2 systems (grid path, Ewald kmax 6) for 5 + 15 cycles at 1e6 Pa."""
+    """Two independent CO2 systems in charged ITQ-29 (grid path, Ewald kmax 6) for 5 + 15 cycles at 1e5 Pa."""
     print("case: gcmc CO2 replay", flush=True)
-    spec = w.forcefield_config1()
+    spec = w.forcefield_zeolite()
     types = [spec.index("C_co2"), spec.index("O_co2")]
-    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types)
-    comp = w.co2_component(spec)
-    params = w.mc_params(pressure_pa=1e6, seed=5, max_molecules=128, production_cycles=15)
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    comp = w.co2_reference(spec)
+    params = w.mc_params(pressure_pa=1e5, seed=5, max_molecules=128, production_cycles=15)
     systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
                          capacity=4 * 128) for _ in range(2)]
     mc = ra.Mc(gpu_ctx, systems, [comp], params)
     mc.run(5, sample=False, graph=False)
     mc.run(15, sample=True, graph=True)
     for sidx in range(2):
-        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (6, 6, 6), host, hcoul, dims, (5, 15))
-        assert g["accepted"][0] > 0
+        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (6, 6, 6), host, hcoul, dims, unit, (5, 15))
+        assert g["accepted"][0] > 0 and g["accepted"][2] > 0 and g["accepted"][3] > 0
     # drift: running energy equals the energy recomputed from scratch
     tot = mc.total_energy()
     for sidx in range(2):
@@ -131,13 +146,13 @@ def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx):
 
 
 def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
-    """CO2/N2 80:20 with a tabulated first-bead proposal (random positive voxel probabilities)."""
+    """CO2/N2 80:20 in charged ITQ-29 with a tabulated first-bead proposal (random positive voxel probabilities)."""
     print("case: gcmc mixture + proposal replay", flush=True)
-    spec = w.forcefield_co2_n2()
+    spec = w.forcefield_zeolite([("N_n2", 36.0, 3.31, -0.482), ("N_com", 0.0, 1.0, 0.964)])
     types = [spec.index(n) for n in ("C_co2", "O_co2", "N_n2", "N_com")]
-    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types)
-    comps = [w.co2_component(spec, 0.8), w.n2_component(spec, 0.2)]
-    params = w.mc_params(pressure_pa=2e6, seed=77, max_molecules=128, production_cycles=10)
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    comps = [w.co2_reference(spec, 0.8), w.n2_component(spec, 0.2)]
+    params = w.mc_params(pressure_pa=5e5, seed=77, max_molecules=128, production_cycles=10)
     s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
                   capacity=4 * 128)
     mc = ra.Mc(gpu_ctx, [s], comps, params)
@@ -147,9 +162,10 @@ def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
     mc.set_proposal(0, n3, prob)
     mc.run(4, sample=False, graph=True)
     mc.run(10, sample=True, graph=True)
-    compare_with_oracle(mc, 0, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, (4, 10), proposal=(n3, prob))
+    compare_with_oracle(mc, 0, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, unit, (4, 10),
+                        proposal=(n3, prob))
     st = mc.state(0)
-    assert st["n_per_component"][0] > 0
+    assert st["n_per_component"][0] > 0 and st["n_molecules"] == sum(st["n_per_component"])
     mc.close()
 
 
@@ -157,11 +173,11 @@ def test_graph_replay_equals_eager(gpu_ctx):
     """Replaying one captured MC cycle gives the same bits as launching it eagerly, with host synchronisation and
     state queries between the replays."""
     print("case: graph vs eager", flush=True)
-    spec = w.forcefield_config1()
+    spec = w.forcefield_zeolite()
     types = [spec.index("C_co2"), spec.index("O_co2")]
-    fw, ff, f, dims, grids, coul, host, hcoul = small_charged_setup(gpu_ctx, spec, types, spacing=0.4)
-    comp = w.co2_component(spec)
-    params = w.mc_params(pressure_pa=1e6, seed=8, max_molecules=64, production_cycles=6)
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types, spacing=0.4)
+    comp = w.co2_reference(spec)
+    params = w.mc_params(pressure_pa=1e5, seed=8, max_molecules=64, production_cycles=6)
     runs = []
     for graph in (False, True):
         systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul,
@@ -174,6 +190,7 @@ def test_graph_replay_equals_eager(gpu_ctx):
         runs.append((trace, [mc.adsorbates(i)[0] for i in range(3)]))
         mc.close()
     assert runs[0][0] == runs[1][0]
+    assert all(len(p) > 0 for p in runs[0][1])
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
 

@@ -162,6 +162,10 @@ def test_ewald_delta_accept_matches_oracle(gpu_ctx):
     assert np.allclose(d, do, rtol=1e-10, atol=1e-6), (d, do)
     s.ewald_accept()
     tot2, stored2 = oracle.ewald_total(fw.cell, fw.rect, alpha, kmax, fw.pos, fw.charge, pos, q, np.ones(len(q)), m)
+    # the accepted structure factor carries the new molecule; the adsorbate list (self and exclusion terms) is the
+    # caller's and still holds the old atoms until it is set again
+    assert np.allclose(s.ewald_energy()[0], tot2[0], rtol=1e-10, atol=1e-6)
+    s.set_adsorbates(pos, t, q, m)
     assert np.allclose(s.ewald_energy(), tot2, rtol=1e-10, atol=1e-6)
     # displacement of molecule 4
     sel = m == 4
