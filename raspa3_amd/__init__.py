@@ -13,6 +13,19 @@ import numpy as np
 from . import _lib as L
 from ._lib import RGError, lib
 
+# Internal units of the reference (src/foundationkit/units.ixx:25-60, src/raspakit/units.cpp:15-100): length 1 A,
+# time 1 ps, mass 1 u; energies in 10 J/mol.
+_E_UNIT = 1.6605402e-27 * 1e-10 * 1e-10 / (1e-12 * 1e-12)
+ENERGY_TO_KELVIN = _E_UNIT * 6.0221419947e23 / 8.314464919
+KB = 1.380650324e-23 / _E_UNIT           # Boltzmann constant in internal energy units per K
+PRESSURE_TO_PA = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+
+
+def beta(temperature):
+    """1 / (kB T) in inverse internal energy units."""
+    return 1.0 / (KB * temperature)
+
+
 __all__ = ["Context", "ForceField", "Framework", "Grid", "box", "grid_dims", "RGError", "lib"]
 
 

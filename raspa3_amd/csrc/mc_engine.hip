@@ -252,17 +252,10 @@ __device__ void waveAcceptEwald(const SysDev& S, int lane)
   for (int k = lane; k < S.nk; k += 64) S.Sk[k] = S.St[k];
 }
 
+
 // ---------------------------------------------------------------------------------------------------------------
 // insertion growth and deletion retrace (rigid molecules)
 // ---------------------------------------------------------------------------------------------------------------
-
-struct WaveScratch
-{
-  double pos[64 * 3 * (MC_MAXATOMS - 1) > 64 * 3 ? 64 * 3 * (MC_MAXATOMS - 1) : 64 * 3];
-  double at[4 * 2 * MC_MAXATOMS];
-  double2 tab[2 * MC_MAXATOMS * 3 * (EW_MAXK + 1)];
-  double mol[3 * MC_MAXATOMS];
-};
 
 // First-bead position of trial j: uniform in the cell or drawn from the proposal (alias table over voxels, uniform
 // inside the voxel).  Returns log(1 / (V q)) = -log(nvox p_v) (0 for the uniform distribution).
@@ -733,13 +726,15 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
   }
 }
 
+// One wavefront per system, blockDim.x / 64 (1, 2 or 4) systems per workgroup: with few systems, one per workgroup
+// spreads them over all compute units.  Dynamic LDS: one WaveScratch per wavefront.
 __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
                                               McSys* __restrict__ Yarr, int nsys, int cycles, int sample)
 {
-  __shared__ WaveScratch scratch[4];
+  extern __shared__ WaveScratch scratch[];
   const McRun& R = *Rp;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int sys = blockIdx.x * 4 + w;
+  const int sys = blockIdx.x * (blockDim.x >> 6) + w;
   if (sys >= nsys) return;
   SysDev& S = Sarr[sys];
   McSys& Y = Yarr[sys];

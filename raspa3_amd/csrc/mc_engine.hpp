@@ -86,4 +86,13 @@ struct McRun
   Comp comp[MC_MAXCOMP];
 };
 
+// LDS of one wavefront in the Monte Carlo kernels: trial atom positions, Ewald eik tables and the grown molecule.
+struct WaveScratch
+{
+  double pos[64 * 3 * (MC_MAXATOMS - 1) > 64 * 3 ? 64 * 3 * (MC_MAXATOMS - 1) : 64 * 3];
+  double at[4 * 2 * MC_MAXATOMS];
+  double2 tab[2 * MC_MAXATOMS * 3 * (EW_MAXK + 1)];
+  double mol[3 * MC_MAXATOMS];
+};
+
 }  // namespace rg

@@ -32,6 +32,7 @@ struct rg_mc
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphSample = -1;
+  int computeUnits = 256;
 };
 
 namespace
@@ -127,6 +128,9 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
   rg_mc* mc = new rg_mc();
   mc->ctx = ctx;
   mc->device = ctx->device;
+  if (hipDeviceGetAttribute(&mc->computeUnits, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+      mc->computeUnits <= 0)
+    mc->computeUnits = 256;
   mc->systems.assign(systems, systems + n_systems);
   rg_status st = buildRun(ctx, ff, n_components, components, params, mc->run);
   if (st != RG_OK)
@@ -256,12 +260,16 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
   RG_ENTER(ctx);
   const int nsys = static_cast<int>(mc->systems.size());
-  const dim3 grid((nsys + 3) / 4), block(256);
+  // systems per workgroup: one while there are fewer systems than 2 per compute unit, so that they spread over the
+  // whole chip, up to four for large batches
+  const int wpb = nsys >= 4 * mc->computeUnits ? 4 : (nsys >= 2 * mc->computeUnits ? 2 : 1);
+  const dim3 grid((nsys + wpb - 1) / wpb), block(64 * wpb);
+  const size_t lds = wpb * sizeof(WaveScratch);
   if (!use_graph)
   {
     for (int c = 0; c < cycles; ++c)
     {
-      hipLaunchKernelGGL(k_gcmc, grid, block, 0, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+      hipLaunchKernelGGL(k_gcmc, grid, block, lds, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
       RG_HIP(ctx, hipGetLastError());
     }
   }
@@ -275,7 +283,7 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
       mc->graphExec = nullptr;
       mc->graph = nullptr;
       RG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      hipLaunchKernelGGL(k_gcmc, grid, block, 0, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+      hipLaunchKernelGGL(k_gcmc, grid, block, lds, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
       hipError_t e = hipStreamEndCapture(ctx->stream, &mc->graph);
       if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
       RG_HIP(ctx, hipGraphInstantiate(&mc->graphExec, mc->graph, nullptr, nullptr, 0));
