@@ -1,17 +1,27 @@
-"""Benchmark: interpolation-grid build rate of the framework-energy path on one MI355X.
+"""Benchmark of the framework-energy path on one MI355X.
 
-Workload (the configuration the metric is quoted on): CO2 grids with charges — triclinic cell a = b = 40, c = 30 A,
-gamma = 100 deg, 4095 framework sites (1365 Si +2.05 e, 2730 O -1.025 e), LJ on O (44 K, 3.3 A) with
+Headline (value): interpolation-grid build rate.  Workload: CO2 grids with charges — triclinic cell a = b = 40,
+c = 30 A, gamma = 100 deg, 4095 framework sites (1365 Si +2.05 e, 2730 O -1.025 e), LJ on O (44 K, 3.3 A) with
 Lorentz-Berthelot mixing to C_co2 (27 K, 2.80 A) and O_co2 (79 K, 3.05 A), Ewald alpha = 0.265 A^-1, 12 A cutoff,
 h = 0.1 A -> 394 x 394 x 300 points per grid.  One step builds the three grids (C_co2 LJ, O_co2 LJ, unit-charge
 Ewald real space: 139.7M grid points, 8.94 GB FP64) in one fused pass with inputs resident in HBM.
-
 value = grid points per second (framework-energy evaluations with their tricubic derivatives) over the timed steps.
 roofline = FP64 vector FLOP/s of the build kernel against the MI355X FP64 vector peak (78.6 TFLOP/s), with the
 algorithmic FLOP count of the frozen convention: 79 FLOP per in-cutoff LJ pair and 86 FLOP (+3 transcendentals,
 not counted) per in-cutoff Ewald real-space pair, triclinic cell.
 cpu_baseline = the FP64 oracle (the reference's loop order and arithmetic restated in C++) on the host cores, on a
 bounded sample of z-planes of the same three grids.
+
+Second half of the metric, in the same JSON line:
+* trial_energies: Widom insertions through the device Monte Carlo kernel — united-atom CH4 in the configuration-0
+  framework (grid at 0.2 A) and rigid CO2 in the configuration-1 framework (grids at 0.1 A, Ewald kmax 8); trial
+  site energies per second, with the grid-gather roofline (512 B per interpolated site).
+* gcmc: CO2 GCMC at 300 K and 1e5 Pa in charged ITQ-29 (3 x 3 x 3 cells, the reference's test structure and zeolite
+  force field), many independent systems in one launch per cycle (hipGraph replay), plain CBMC against the learned
+  first-bead proposal: accepted insertions per second over the production cycles, loading and isosteric heat with
+  their standard errors (over systems), and whether they agree within 2 sigma.
+* mlp: training throughput of the proposal network (BF16 MFMA) on 2^21 points of the configuration-1 CO2 grids at
+  300 K, samples/s and MFMA TFLOP/s at 1.08 MFLOP per training sample.
 """
 import argparse
 import json
@@ -25,12 +35,24 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
 FLOP_LJ_TRIC = 79
 FLOP_EWALD_TRIC = 86
+GRID_BYTES_PER_SITE = 512      # 8 corner chunks of 64 B per tricubic interpolation
+MLP_FLOP_PER_SAMPLE = 1.08e6   # forward 2 * 180.5k + backward 4 * 180.5k (GEMM work of a training sample)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 64))
 
 
 def incutoff_counts(fw, lj_types, rc, nsample=20000, seed=0):
@@ -67,11 +89,7 @@ def cpu_baseline(fw, spec, dims, target_s=12.0):
     import oracle
 
     pairs = oracle.pairs_from_spec(spec)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 64))
+    cores = host_cores()
     grids = [(0, spec.index("C_co2")), (0, spec.index("O_co2")), (3, 0)]
 
     def run(planes):
@@ -90,19 +108,7 @@ def cpu_baseline(fw, spec, dims, target_s=12.0):
                        f"{dims[0]}x{dims[1]} points x 3 grids = {pts} points in {t:.2f} s on {cores} threads")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build path shards over "
-              "z-slabs across GPUs by design, see DESIGN.md, but only the single-GPU case is run)", file=sys.stderr)
-        sys.exit(2)
-
+def bench_grids(ctx, args):
     import torch
 
     import raspa3_amd as ra
@@ -112,14 +118,13 @@ def main():
     fw, spec = w.framework_config1(), w.forcefield_config1()
     dims = ra.grid_dims(fw.cell, 0.1)
     npts = dims[0] * dims[1] * dims[2]
-    ctx = ra.Context(0)
     ff = ra.ForceField(ctx, spec)
     f = ra.Framework(ctx, fw)
     grids = [ra.Grid(ctx, fw.cell, fw.rect, dims) for _ in range(3)]
     specs = [(grids[0], RG_GRID_LENNARD_JONES, spec.index("C_co2"), spec.rc_framework),
              (grids[1], RG_GRID_LENNARD_JONES, spec.index("O_co2"), spec.rc_framework),
              (grids[2], RG_GRID_EWALD_REAL, 0, spec.rc_coulomb)]
-    log(f"workload: 3 grids {dims} ({3 * npts} points), {fw.n} framework sites")
+    log(f"grids: 3 x {dims} ({3 * npts} points), {fw.n} framework sites")
     for _ in range(max(1, args.warmup)):
         ra.build_grids(ctx, ff, f, specs)
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -138,7 +143,7 @@ def main():
     ms_per_step = (t1 - t0) * 1000.0 / args.steps
     kernel_ms = ev0.elapsed_time(ev1) / args.steps
     value = 3 * npts / (ms_per_step / 1000.0)
-    log(f"ms/step {ms_per_step:.3f} (events {kernel_ms:.3f}); {value:.4e} grid points/s")
+    log(f"grids: ms/step {ms_per_step:.3f} (events {kernel_ms:.3f}); {value:.4e} grid points/s")
 
     n_all, n_lj = incutoff_counts(fw, [spec.index("O")], spec.rc_framework)
     flop = npts * (2 * FLOP_LJ_TRIC * n_lj + FLOP_EWALD_TRIC * n_all)
@@ -149,10 +154,286 @@ def main():
                 flop_per_launch=float(flop), incutoff_pairs_per_point=dict(lj_active=round(n_lj, 2),
                                                                            charged=round(n_all, 2)),
                 kernel_ms=round(kernel_ms, 4))
+    state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, dims=dims)
+    return value, ms_per_step, npts, roof, state
+
+
+def bench_trial_energies(ctx, g1, args):
+    """Widom insertions through the device kernel: trial site energies per second."""
+    import raspa3_amd as ra
+    from raspa3_amd import workloads as w
+    from raspa3_amd._lib import RG_GRID_LENNARD_JONES
+
+    out = {}
+    # configuration 0: CH4 in the 288-site framework, grid at 0.2 A, 10 first-bead trials per insertion
+    fw0, spec0 = w.framework_config0(), w.forcefield_config0()
+    ff0 = ra.ForceField(ctx, spec0)
+    f0 = ra.Framework(ctx, fw0)
+    d0 = ra.grid_dims(fw0.cell, 0.2)
+    g0 = ra.Grid(ctx, fw0.cell, fw0.rect, d0)
+    ra.build_grids(ctx, ff0, f0, [(g0, RG_GRID_LENNARD_JONES, spec0.index("CH4"), 12.0)])
+    s0 = ra.System(ctx, ff0, fw0.cell, fw0.rect, framework=f0, grids={spec0.index("CH4"): g0}, kmax=(0, 0, 0),
+                   capacity=64)
+    comp0 = w.methane_component(spec0)
+    p0 = w.mc_params(seed=7)
+    n = args.widom_insertions
+    ra.widom(ctx, s0, comp0, p0, 0, 4096)
+    t = time.perf_counter()
+    lw, _, _ = ra.widom(ctx, s0, comp0, p0, 4096, n)
+    dt = time.perf_counter() - t
+    sites = n * p0["k_first"]
+    out["ch4_config0"] = dict(insertions=n, seconds=round(dt, 4), sites_per_s=sites / dt,
+                              widom_weight=float(np.exp(lw).mean()),
+                              roofline=dict(bound="hbm", achieved=sites / dt * GRID_BYTES_PER_SITE / 1e9,
+                                            peak=HBM_PEAK_GBS, unit="GB/s",
+                                            frac=sites / dt * GRID_BYTES_PER_SITE / 1e9 / HBM_PEAK_GBS))
+    log(f"trial energies CH4 config 0: {sites / dt:.4e} sites/s")
+    # configuration 1: rigid CO2 on the three configuration-1 grids, Ewald Fourier difference at kmax 8
+    fw1, spec1 = g1["fw"], g1["spec"]
+    s1 = ra.System(ctx, g1["ff"], fw1.cell, fw1.rect, framework=g1["f"],
+                   grids={spec1.index("C_co2"): g1["grids"][0], spec1.index("O_co2"): g1["grids"][1]},
+                   coulomb_grid=g1["grids"][2], kmax=(8, 8, 8), capacity=64)
+    comp1 = w.co2_component(spec1)
+    p1 = w.mc_params(seed=9)
+    n1 = max(1024, n // 8)
+    ra.widom(ctx, s1, comp1, p1, 0, 1024)
+    t = time.perf_counter()
+    ra.widom(ctx, s1, comp1, p1, 1024, n1)
+    dt = time.perf_counter() - t
+    sites = n1 * (p1["k_first"] + 2 * p1["k_orient"])
+    out["co2_config1"] = dict(insertions=n1, seconds=round(dt, 4), sites_per_s=sites / dt,
+                              ewald_deltas_per_s=n1 / dt, n_kvectors=s1.info()["nk"])
+    log(f"trial energies CO2 config 1: {sites / dt:.4e} sites/s, {n1 / dt:.4e} Ewald deltas/s")
+    out["unit"] = "trial site energies/s"
+    return out
+
+
+def gcmc_stats(mc, nsys, base, temperature):
+    """Loading (per system, standard error over systems), isosteric heat from the pooled fluctuation formula
+    (jackknife over systems), accepted insertions during production."""
+    import raspa3_amd as ra
+
+    rows, acc, tri, per = [], 0, 0, []
+    for s in range(nsys):
+        st = mc.state(s)
+        c = st["block_count"].sum()
+        rows.append([c, st["block_energy"].sum(), st["block_n"][:, 0].sum(), st["block_energy_n"][:, 0].sum(),
+                     st["block_nn"][:, 0, 0].sum()])
+        per.append(st["block_n"][:, 0].sum() / c)
+        acc += st["accepted"][0] - base[s][0]
+        tri += st["trials"][0] - base[s][1]
+    rows = np.array(rows)
+    per = np.array(per)
+
+    def qst(r):
+        cnt, U, N, UN, NN = r
+        cov = UN / cnt - U / cnt * N / cnt
+        var = NN / cnt - (N / cnt) ** 2
+        return (-cov / var * ra.ENERGY_TO_KELVIN + temperature) * 8.314462618e-3  # kJ/mol
+
+    tot = rows.sum(0)
+    jk = np.array([qst(tot - r) for r in rows])
+    return dict(loading=float(per.mean()), loading_err=float(per.std(ddof=1) / np.sqrt(nsys)),
+                qst_kJmol=float(qst(tot)),
+                qst_err=float(np.sqrt((nsys - 1) / nsys * ((jk - jk.mean()) ** 2).sum())),
+                accepted_insertions=int(acc), insertion_trials=int(tri))
+
+
+def bench_gcmc(ctx, args):
+    """Plain CBMC against the learned first-bead proposal, CO2 in charged ITQ-29 at 300 K and 1e5 Pa."""
+    import torch
+
+    import raspa3_amd as ra
+    from raspa3_amd import workloads as w
+    from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
+
+    T, P = 300.0, 1e5
+    spec = w.forcefield_zeolite()
+    fw, unit = w.framework_itq29(spec, 3)
+    ff = ra.ForceField(ctx, spec)
+    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw.pos[:72], fw.type[:72], fw.charge[:72]))
+    dims = ra.grid_dims(unit, 0.2)
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
+    coul = ra.Grid(ctx, unit, True, dims)
+    ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
+                   [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    fb = ra.Framework(ctx, fw)
+    _, kmax = ra.ewald_parameters(1e-6, 12.0, np.diag(fw.cell))
+    comp = w.co2_reference(spec)
+    nsys = args.gcmc_systems
+    stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def run(label, seed, proposal=None):
+        systems = [ra.System(ctx, ff, fw.cell, True, framework=fb, grids=grids, coulomb_grid=coul,
+                             kmax=tuple(kmax), capacity=4 * 256) for _ in range(nsys)]
+        params = w.mc_params(temperature=T, pressure_pa=P, seed=seed, max_molecules=256,
+                             production_cycles=args.gcmc_production)
+        mc = ra.Mc(ctx, systems, [comp], params)
+        if proposal is not None:
+            for s in range(nsys):
+                mc.set_proposal(s, proposal[0], proposal[1])
+        mc.run(args.gcmc_equilibration, sample=False, graph=True)
+        base = [(mc.state(s)["accepted"][0], mc.state(s)["trials"][0]) for s in range(nsys)]
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ctx.sync()
+        ev0.record(stream)
+        t = time.perf_counter()
+        mc.run(args.gcmc_production, sample=True, graph=True)
+        dt = time.perf_counter() - t
+        ev1.record(stream)
+        ctx.sync()
+        r = gcmc_stats(mc, nsys, base, T)
+        tot = mc.total_energy()
+        r["max_energy_drift"] = float(max(abs(tot[s] - mc.state(s)["energy"]) for s in range(nsys)))
+        r.update(seconds=round(dt, 4), accepted_insertions_per_s=r["accepted_insertions"] / dt,
+                 insertion_acceptance=r["accepted_insertions"] / max(1, r["insertion_trials"]),
+                 kernel_ms_per_cycle=ev0.elapsed_time(ev1) / args.gcmc_production)
+        log(f"gcmc {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, N = {r['loading']:.3f} +- "
+            f"{r['loading_err']:.3f}, Qst = {r['qst_kJmol']:.3f} +- {r['qst_err']:.3f} kJ/mol")
+        mc.close()
+        for s in systems:
+            s.close()
+        return r
+
+    plain = run("plain", 1001)
+    # learned proposal: orientation-averaged Boltzmann factors of CO2 from the grids at 2^21 random points, tempered
+    # (trained at tau * beta), tabulated on voxel centres with a uniform floor
+    s0 = ra.System(ctx, ff, fw.cell, True, framework=fb, grids=grids, coulomb_grid=coul, kmax=tuple(kmax), capacity=64)
+    rng = np.random.default_rng(5)
+    frac = rng.random((1 << 21, 3))
+    t = time.perf_counter()
+    lw = ra.molecule_weights(ctx, s0, comp, T, frac, 16, 11)
+    t_data = time.perf_counter() - t
+    beta = ra.beta(T)
+    u_eff = np.where(np.isfinite(lw), -lw / beta, 1e7)
+    net = ra.Mlp(ctx, seed=3)
+    t = time.perf_counter()
+    losses = net.train(frac, u_eff, args.tau * beta, batch=4096, epochs=args.mlp_epochs, lr=1e-3, seed=9)
+    t_train = time.perf_counter() - t
+    n3 = [args.voxels] * 3
+    prob = net.proposal(n3, args.floor)
+    learned = run("learned", 1002, (n3, prob))
+    net.close()
+
+    def within(a, b, k):
+        return abs(learned[a] - plain[a]) <= k * np.hypot(learned[b], plain[b])
+
+    return dict(workload=f"CO2 (reference rigid model) in charged ITQ-29 3x3x3 (1944 sites), {T:g} K, {P:g} Pa, "
+                         f"{nsys} systems, {args.gcmc_equilibration} + {args.gcmc_production} cycles, grids at 0.2 A "
+                         f"on the unit cell, Ewald kmax {list(kmax)}, 10 first-bead + 10 orientation trials",
+                plain=plain, learned=learned,
+                learned_over_plain=learned["accepted_insertions_per_s"] / plain["accepted_insertions_per_s"],
+                loading_within_2sigma=bool(within("loading", "loading_err", 2.0)),
+                qst_within_2sigma=bool(within("qst_kJmol", "qst_err", 2.0)),
+                proposal=dict(training_points=len(frac), orientations=16, tau=args.tau, floor=args.floor,
+                              voxels=n3, epochs=args.mlp_epochs, final_loss=float(losses[-1]),
+                              data_seconds=round(t_data, 4), train_seconds=round(t_train, 4)),
+                unit="accepted insertions/s")
+
+
+def bench_mlp(ctx, g1, args):
+    """Proposal-network training throughput on configuration-1 CO2 grid data (2^21 points, 300 K)."""
+    import raspa3_amd as ra
+    from raspa3_amd import workloads as w
+
+    fw, spec = g1["fw"], g1["spec"]
+    s = ra.System(ctx, g1["ff"], fw.cell, fw.rect, framework=g1["f"],
+                  grids={spec.index("C_co2"): g1["grids"][0], spec.index("O_co2"): g1["grids"][1]},
+                  coulomb_grid=g1["grids"][2], kmax=(0, 0, 0), capacity=64)
+    comp = w.co2_component(spec)
+    frac = np.random.default_rng(1).random((1 << 21, 3))
+    beta = ra.beta(300.0)
+    lw = ra.molecule_weights(ctx, s, comp, 300.0, frac, 16, 3)
+    u = np.where(np.isfinite(lw), -lw / beta, 1e7)
+    net = ra.Mlp(ctx, seed=1)
+    net.train(frac[:1 << 16], u[:1 << 16], beta, batch=args.mlp_batch, epochs=1, lr=1e-3, seed=1)
+    t = time.perf_counter()
+    losses = net.train(frac, u, beta, batch=args.mlp_batch, epochs=args.mlp_epochs, lr=1e-3, seed=2)
+    dt = time.perf_counter() - t
+    samples = (len(frac) // args.mlp_batch) * args.mlp_batch * args.mlp_epochs
+    tflops = samples / dt * MLP_FLOP_PER_SAMPLE / 1e12
+    net.close()
+    log(f"mlp: {samples / dt:.4e} samples/s, {tflops:.2f} TFLOP/s BF16 MFMA")
+    return dict(samples_per_s=samples / dt, batch=args.mlp_batch, epochs=args.mlp_epochs, seconds=round(dt, 4),
+                finite_fraction=float(np.isfinite(lw).mean()), losses=[float(x) for x in losses],
+                roofline=dict(bound="mfma", achieved=tflops, peak=BF16_PEAK_TFLOPS, unit="TFLOP/s",
+                              frac=tflops / BF16_PEAK_TFLOPS))
+
+
+def cpu_baseline_gcmc(target_s=10.0):
+    """The oracle's plain GCMC replay of one system (one host thread), same workload as the GPU run."""
+    import oracle
+    from raspa3_amd import workloads as w
+
+    spec = w.forcefield_zeolite()
+    fw, unit = w.framework_itq29(spec, 3)
+    dims = w.grid_dims(unit, 0.2)[0]
+    pairs = oracle.pairs_from_spec(spec)
+    g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
+                              spec.alpha, nthreads=host_cores())
+         for t in (spec.index("C_co2"), spec.index("O_co2"))}
+    coul = oracle.grid_build(unit, True, dims, 3, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, 0, 12.0, 12.0,
+                             spec.alpha, nthreads=host_cores())
+    _, kmax = oracle.ewald_auto(1e-6, 12.0, np.diag(fw.cell))
+    mc = oracle.MC(spec, fw, [w.co2_reference(spec)], w.mc_params(seed=3, max_molecules=256),
+                   kmax=tuple(int(k) for k in kmax), grids=(g, coul, dims, unit))
+    mc.run(100)  # loads the system part of the way
+    a0 = mc.state()["accepted"][0]
+    cyc, dt = 0, 0.0
+    while dt < target_s:
+        t = time.perf_counter()
+        mc.run(20)
+        dt += time.perf_counter() - t
+        cyc += 20
+    acc = mc.state()["accepted"][0] - a0
+    return dict(value=acc / dt, unit="accepted insertions/s", cores=1, kind="port",
+                sample=f"oracle plain GCMC, 1 system, cycles 100-{100 + cyc} ({acc} accepted insertions in "
+                       f"{dt:.1f} s, N = {mc.state()['n_molecules']}); systems are independent, so C host cores "
+                       f"give at most C times this")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--grids-only", action="store_true", help="measure the headline grid build only")
+    ap.add_argument("--widom-insertions", type=int, default=1 << 20)
+    ap.add_argument("--gcmc-systems", type=int, default=256)
+    ap.add_argument("--gcmc-equilibration", type=int, default=600)
+    ap.add_argument("--gcmc-production", type=int, default=400)
+    ap.add_argument("--tau", type=float, default=0.5)
+    ap.add_argument("--floor", type=float, default=0.1)
+    ap.add_argument("--voxels", type=int, default=64)
+    ap.add_argument("--mlp-epochs", type=int, default=4)
+    ap.add_argument("--mlp-batch", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build path shards over "
+              "z-slabs across GPUs by design, see DESIGN.md, but only the single-GPU case is run)", file=sys.stderr)
+        sys.exit(2)
+
+    import raspa3_amd as ra
+
+    ctx = ra.Context(0)
+    value, ms_per_step, npts, roof, g1 = bench_grids(ctx, args)
+    extra = {}
+    if not args.grids_only:
+        extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
+        extra["mlp"] = bench_mlp(ctx, g1, args)
+        extra["gcmc"] = bench_gcmc(ctx, args)
     base = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline(fw, spec, dims, args.cpu_seconds)
+        base = cpu_baseline(g1["fw"], g1["spec"], g1["dims"], args.cpu_seconds)
         log(f"cpu baseline {base['value']:.4e} points/s ({base['sample']})")
+        if "gcmc" in extra:
+            extra["gcmc"]["cpu_baseline"] = cpu_baseline_gcmc()
+            log(f"cpu baseline gcmc {extra['gcmc']['cpu_baseline']['value']:.4e} accepted insertions/s")
+    fw, spec = g1["fw"], g1["spec"]
     out = dict(metric="framework-energy evals/s; GCMC accepted insertions/s at matched loading",
                value=value, unit="grid points/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
@@ -161,7 +442,7 @@ def main():
                                     "C_co2 LJ + O_co2 LJ + Ewald-real grids at h=0.1 A (394x394x300 each), "
                                     "fused build", grid_points_per_step=3 * npts, framework_sites=fw.n,
                            cutoff=spec.rc_framework, alpha=spec.alpha),
-               roofline=roof, cpu_baseline=base)
+               roofline=roof, cpu_baseline=base, **extra)
     print(json.dumps(out), flush=True)
     ctx.close()
 

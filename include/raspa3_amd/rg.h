@@ -44,6 +44,10 @@ typedef struct rg_grid rg_grid;
 rg_status rg_ctx_create(int device, rg_ctx** out);
 /* The *_destroy functions wait for the device and do not use the context, so objects may be released after it. */
 rg_status rg_ctx_destroy(rg_ctx* ctx);
+/* Decisions of the online autotuner (launch shapes timed on the first calls of each problem size, see
+ * raspa3_amd/csrc/autotune.hpp), one line per tuned key; writes at most len - 1 characters and a terminating NUL into
+ * buf (if not NULL) and returns the full length.  No reference counterpart. */
+size_t rg_autotune_summary(char* buf, size_t len);
 /* Last error message of this context (empty string if none).  Valid until the next call. */
 const char* rg_last_error(const rg_ctx* ctx);
 /* Blocks until all work issued on the context's stream has finished. */

@@ -319,6 +319,14 @@ def widom(ctx, system, comp, params, first, count):
     return lw, u, ew
 
 
+def autotune_summary():
+    """Launch configurations chosen by the library's online autotuner, one line per tuned problem."""
+    n = lib().rg_autotune_summary(None, 0)
+    buf = C.create_string_buffer(n + 1)
+    lib().rg_autotune_summary(buf, n + 1)
+    return buf.value.decode()
+
+
 def molecule_weights(ctx, system, comp, temperature, frac, n_orient=16, seed=1):
     """log of the orientation-averaged Boltzmann factor of a rigid molecule with its first bead at each point
     (fractional coordinates of the system's box), framework energy from the grids."""

@@ -140,6 +140,7 @@ fp = C.POINTER(C.c_float)
 
 SIGNATURES.update({
     "rg_mlp_num_params": (C.c_size_t, []),
+    "rg_autotune_summary": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "rg_mlp_create": (C.c_int, [vp, C.c_ulonglong, C.POINTER(vp)]),
     "rg_mlp_destroy": (C.c_int, [vp]),
     "rg_mlp_get_params": (C.c_int, [vp, vp, fp]),

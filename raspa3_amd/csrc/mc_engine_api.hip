@@ -4,6 +4,7 @@
 #include <cstring>
 #include <vector>
 
+#include "autotune.hpp"
 #include "mc_engine.hpp"
 
 namespace rg
@@ -32,6 +33,7 @@ struct rg_mc
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphSample = -1;
+  int graphWpb = 0;
   int computeUnits = 256;
 };
 
@@ -260,36 +262,50 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
   RG_ENTER(ctx);
   const int nsys = static_cast<int>(mc->systems.size());
-  // systems per workgroup: one while there are fewer systems than 2 per compute unit, so that they spread over the
-  // whole chip, up to four for large batches
-  const int wpb = nsys >= 4 * mc->computeUnits ? 4 : (nsys >= 2 * mc->computeUnits ? 2 : 1);
-  const dim3 grid((nsys + wpb - 1) / wpb), block(64 * wpb);
-  const size_t lds = wpb * sizeof(WaveScratch);
+  // Systems per workgroup (1, 2 or 4 wavefronts) is tuned online: while the key is open, cycles of the real run
+  // are launched eagerly with each candidate and timed.  Every candidate computes the same trajectory (one
+  // wavefront per system in every shape), so the choice only affects speed.
+  static const int kWpb[3] = {1, 2, 4};
+  auto launch = [&](int wpb)
+  {
+    hipLaunchKernelGGL(k_gcmc, dim3((nsys + wpb - 1) / wpb), dim3(64 * wpb), wpb * sizeof(WaveScratch), ctx->stream,
+                       mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+    return hipGetLastError();
+  };
+  Autotuner& tuner = Autotuner::instance();
+  const std::string key = "k_gcmc/systems=" + std::to_string(nsys) + "/cu=" + std::to_string(mc->computeUnits);
+  int done = 0;
+  for (int cand; done < cycles && (cand = tuner.next(key, 3, 2)) >= 0; ++done)
+  {
+    float ms = 0.0f;
+    RG_HIP(ctx, timeLaunch(ctx->stream, [&] { return launch(kWpb[cand]); }, ms));
+    tuner.report(key, cand, ms);
+  }
+  const int best = tuner.best(key);
+  const int wpb = kWpb[best >= 0 ? best : 0];
   if (!use_graph)
   {
-    for (int c = 0; c < cycles; ++c)
-    {
-      hipLaunchKernelGGL(k_gcmc, grid, block, lds, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
-      RG_HIP(ctx, hipGetLastError());
-    }
+    for (int c = done; c < cycles; ++c) RG_HIP(ctx, launch(wpb));
   }
-  else
+  else if (done < cycles)
   {
     // one MC cycle of every system captured once as a graph, replayed per cycle
-    if (!mc->graphExec || mc->graphSample != sample)
+    if (!mc->graphExec || mc->graphSample != sample || mc->graphWpb != wpb)
     {
       if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
       if (mc->graph) (void)hipGraphDestroy(mc->graph);
       mc->graphExec = nullptr;
       mc->graph = nullptr;
       RG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      hipLaunchKernelGGL(k_gcmc, grid, block, lds, ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+      const hipError_t le = launch(wpb);
       hipError_t e = hipStreamEndCapture(ctx->stream, &mc->graph);
+      if (le != hipSuccess) return hipfail(ctx, le, "k_gcmc launch under capture");
       if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
       RG_HIP(ctx, hipGraphInstantiate(&mc->graphExec, mc->graph, nullptr, nullptr, 0));
       mc->graphSample = sample;
+      mc->graphWpb = wpb;
     }
-    for (int c = 0; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
+    for (int c = done; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
   }
   // hand the adsorbate count and net charge back to the systems (their Ewald state and adsorbate arrays are
   // updated in place by the kernel)

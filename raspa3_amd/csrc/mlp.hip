@@ -16,6 +16,7 @@
 #include <cstring>
 #include <vector>
 
+#include "autotune.hpp"
 #include "rg_internal.hpp"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -387,23 +388,32 @@ rg_status forward(rg_mlp* p, int nrow)
   return RG_OK;
 }
 
-// K chunk of the weight-gradient products: enough workgroups to cover the chip, chunks a multiple of 32 rows
-int splitChunk(int n, int tiles)
+// Weight gradient G (M x WIDTH) += A^T dZ over n rows, A stored n x M, split over K chunks of kc rows whose partial
+// products are added atomically.  The chunk is tuned online among 256 .. 4096 rows (results differ only in the
+// order of the FP32 atomic additions, which is unordered anyway).
+rg_status weightGrad(rg_ctx* ctx, int M, int n, const __bf16* A, const __bf16* dZ, float* G)
 {
-  const int want = std::max(1, 512 / tiles);
-  int kc = (n + want - 1) / want;
-  kc = std::max(256, (kc + 31) / 32 * 32);
-  return std::min(kc, n);
-}
-
-// weight gradient G (M x WIDTH) += A^T dZ over n rows, A stored n x M
-void weightGrad(hipStream_t st, int M, int n, const __bf16* A, const __bf16* dZ, float* G)
-{
-  const int tiles = (M / 64) * (WIDTH / 64);
-  const int kc = splitChunk(n, tiles);
-  const dim3 grid(WIDTH / 64, M / 64, (n + kc - 1) / kc);
-  hipLaunchKernelGGL((k_gemm<true, false, EPI_ATOMIC_F32>), grid, dim3(256), 0, st, M, WIDTH, n, kc, A, M, dZ, WIDTH,
-                     nullptr, nullptr, G, nullptr);
+  static const int kChunks[5] = {256, 512, 1024, 2048, 4096};
+  auto launch = [&](int kc)
+  {
+    kc = std::min(kc, n);
+    const dim3 grid(WIDTH / 64, M / 64, (n + kc - 1) / kc);
+    hipLaunchKernelGGL((k_gemm<true, false, EPI_ATOMIC_F32>), grid, dim3(256), 0, ctx->stream, M, WIDTH, n, kc, A, M,
+                       dZ, WIDTH, nullptr, nullptr, G, nullptr);
+    return hipGetLastError();
+  };
+  Autotuner& tuner = Autotuner::instance();
+  const std::string key = "mlp_weight_grad/M=" + std::to_string(M) + "/n=" + std::to_string(n);
+  const int cand = tuner.next(key, 5, 2);
+  if (cand >= 0)
+  {
+    float ms = 0.0f;
+    RG_HIP(ctx, timeLaunch(ctx->stream, [&] { return launch(kChunks[cand]); }, ms));
+    tuner.report(key, cand, ms);
+    return RG_OK;
+  }
+  RG_HIP(ctx, launch(kChunks[tuner.best(key)]));
+  return RG_OK;
 }
 
 // loss and gradients of a batch of n rows (n multiple of 64) already in p->frac / p->U
@@ -417,17 +427,18 @@ rg_status backward(rg_mlp* p, int n, double beta)
   hipLaunchKernelGGL(k_out_backward, dim3(rb), dim3(WIDTH), 0, st, n, p->H3, p->Z3, p->wb + OFF_W4, p->df,
                      p->g + OFF_W4, p->g + OFF_B4, p->dZa);
   // layer 3: dW3 = H2^T dZ3, db3, dZ2 = (dZ3 W3^T) * SiLU'(Z2)
-  weightGrad(st, WIDTH, n, p->H2, p->dZa, p->g + OFF_W3);
+  rg_status ws;
+  if ((ws = weightGrad(p->ctx, WIDTH, n, p->H2, p->dZa, p->g + OFF_W3)) != RG_OK) return ws;
   hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZa, p->g + OFF_B3);
   hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
                      p->dZa, WIDTH, p->wb + OFF_W3, WIDTH, nullptr, p->Z2, nullptr, p->dZb);
   // layer 2
-  weightGrad(st, WIDTH, n, p->H1, p->dZb, p->g + OFF_W2);
+  if ((ws = weightGrad(p->ctx, WIDTH, n, p->H1, p->dZb, p->g + OFF_W2)) != RG_OK) return ws;
   hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZb, p->g + OFF_B2);
   hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
                      p->dZb, WIDTH, p->wb + OFF_W2, WIDTH, nullptr, p->Z1, nullptr, p->dZa);
   // layer 1: dW1 = X^T dZ1, db1
-  weightGrad(st, NIN, n, p->X, p->dZa, p->g + OFF_W1);
+  if ((ws = weightGrad(p->ctx, NIN, n, p->X, p->dZa, p->g + OFF_W1)) != RG_OK) return ws;
   hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZa, p->g + OFF_B1);
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
