@@ -83,12 +83,18 @@ __device__ __forceinline__ void randomRotation(double u1, double u2, double u3, 
 
 // External energy of one site with the whole wavefront (framework direct or grid, plus adsorbates except molecule
 // 'own' and 'skip'); overlap as in the reference (any pair above the criterion, or the grid value above it).
+// withGrid = false leaves out the grid part (the caller interpolates it lane-parallel).
 __device__ double siteExternal(const SysDev& S, int nads, double px, double py, double pz, int type, double q, int own,
-                               int skip, int lane, bool& ov)
+                               int skip, int lane, bool& ov, bool withGrid = true)
 {
   double e = 0.0;
   bool o = false;
   const int gi = S.gridOfType ? S.gridOfType[type] : -1;
+  if (!withGrid && gi >= 0 && nads == 0)
+  {
+    ov = false;
+    return 0.0;
+  }
   if (gi < 0)
   {
     for (int a = lane; a < S.nfw; a += 64)
@@ -122,7 +128,7 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
     if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.aq[a] * q);
   }
   e = rgd::waveSum(e);
-  if (gi >= 0)
+  if (withGrid && gi >= 0)
   {
     const double g = rgd::interpolate(S.grids[gi], px, py, pz);
     o = o || g > S.overlap;
@@ -138,6 +144,24 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
 __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const int* atomIdx, int n, int K,
                                  const double* pos, int own, int skip, int lane, double& Uj, bool& ovj)
 {
+  // grid-typed sites: lane j interpolates the sites of trial set j, so the K sets' gathers are in flight together
+  double gpart = 0.0;
+  bool gov = false;
+  if (lane < K && S.gridOfType)
+  {
+    for (int s = 0; s < n; ++s)
+    {
+      const int a = atomIdx[s];
+      const int gi = S.gridOfType[c.type[a]];
+      if (gi < 0) continue;
+      const double* p = pos + 3 * (lane * n + s);
+      const double g = rgd::interpolate(S.grids[gi], p[0], p[1], p[2]);
+      gov = gov || g > S.overlap;
+      gpart += g;
+      if (S.useCharge && S.coulGrid >= 0) gpart += c.q[a] * rgd::interpolate(S.grids[S.coulGrid], p[0], p[1], p[2]);
+    }
+  }
+  // adsorbate atoms (and framework atoms of types without a grid): one site at a time over the whole wavefront
   Uj = 0.0;
   ovj = false;
   for (int j = 0; j < K; ++j)
@@ -149,13 +173,13 @@ __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const
       const int a = atomIdx[s];
       const double* p = pos + 3 * (j * n + s);
       bool o;
-      u += siteExternal(S, nads, p[0], p[1], p[2], c.type[a], c.q[a], own, skip, lane, o);
+      u += siteExternal(S, nads, p[0], p[1], p[2], c.type[a], c.q[a], own, skip, lane, o, false);
       any = any || o;
     }
     if (lane == j)
     {
-      Uj = u;
-      ovj = any;
+      Uj = u + gpart;
+      ovj = any || gov;
     }
   }
 }

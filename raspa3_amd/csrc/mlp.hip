@@ -35,7 +35,7 @@ enum Epilogue
 {
   EPI_BIAS_SILU = 0,  // Z = acc + bias (FP32 out), H = SiLU(Z) (BF16 out)
   EPI_STORE_F32 = 1,  // C = acc (FP32 out)
-  EPI_DSILU = 2,      // dZ = acc * SiLU'(Zprev) (BF16 out)
+  EPI_DSILU = 2,      // dZ = acc * SiLU'(Zprev) (BF16 out), column sums of dZ added into outF (bias gradient)
   EPI_ATOMIC_F32 = 3  // C += acc (FP32, split-K partial products of the weight gradients)
 };
 
@@ -98,6 +98,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
     }
     __syncthreads();
   }
+  float csum[2] = {0.f, 0.f};  // EPI_DSILU: this lane's share of the column sums of dZ
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -123,9 +124,23 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
         {
           const float z = Zprev[o];
           const float s = sigmoidf(z);
-          outH[o] = (__bf16)(v * s * (1.0f + z * (1.0f - s)));
+          const __bf16 h = (__bf16)(v * s * (1.0f + z * (1.0f - s)));
+          outH[o] = h;
+          csum[j] += float(h);
         }
       }
+  if (EPI == EPI_DSILU)
+  {
+    // bias gradient: the tile's column sums (lanes l, l + 16, l + 32, l + 48 hold column l), added into outF[n]
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+    {
+      float cs = csum[j];
+      cs += __shfl_xor(cs, 16, 64);
+      cs += __shfl_xor(cs, 32, 64);
+      if (lane < 16) atomicAdd(&outF[n0 + wn + 16 * j + lane], cs);
+    }
+  }
 }
 
 // Fourier features of fractional coordinates
@@ -214,36 +229,33 @@ __global__ __launch_bounds__(1024) void k_loss(int n, const float* __restrict__ 
   }
 }
 
-// dw4_j = sum_i H3_ij df_i, db4 = sum_i df_i, dZ3_ij = df_i w4_j SiLU'(Z3_ij)
+// Output layer backward over a chunk of OB_ROWS rows per workgroup, one thread per hidden unit j:
+// dw4_j += sum_i H3_ij df_i, db4 += sum_i df_i, dZ3_ij = df_i w4_j SiLU'(Z3_ij) (BF16), db3_j += sum_i dZ3_ij.
+constexpr int OB_ROWS = 32;
 __global__ void k_out_backward(int n, const __bf16* __restrict__ H, const float* __restrict__ Z,
                                const __bf16* __restrict__ w4, const float* __restrict__ df, float* __restrict__ gw4,
-                               float* __restrict__ gb4, __bf16* __restrict__ dZ)
+                               float* __restrict__ gb4, __bf16* __restrict__ dZ, float* __restrict__ gb3)
 {
-  const int j = threadIdx.x;  // blockDim = WIDTH, one workgroup per 256 rows
-  const int i0 = blockIdx.x * 256;
-  float g = 0.f, gb = 0.f;
+  const int j = threadIdx.x;  // blockDim = WIDTH
+  const int i0 = blockIdx.x * OB_ROWS;
+  const int i1 = min(n, i0 + OB_ROWS);
+  float g = 0.f, gb = 0.f, cs = 0.f;
   const float wj = float(w4[j]);
-  for (int i = i0; i < min(n, i0 + 256); ++i)
+#pragma unroll 4
+  for (int i = i0; i < i1; ++i)
   {
     const size_t o = size_t(i) * WIDTH + j;
     const float d = df[i];
     g += float(H[o]) * d;
     gb += d;
     const float z = Z[o], s = sigmoidf(z);
-    dZ[o] = (__bf16)(d * wj * s * (1.0f + z * (1.0f - s)));
+    const __bf16 h = (__bf16)(d * wj * s * (1.0f + z * (1.0f - s)));
+    dZ[o] = h;
+    cs += float(h);
   }
   atomicAdd(&gw4[j], g);
+  atomicAdd(&gb3[j], cs);
   if (j == 0) atomicAdd(gb4, gb);
-}
-
-// db_j = sum_i dZ_ij
-__global__ void k_colsum(int n, const __bf16* __restrict__ dZ, float* __restrict__ gb)
-{
-  const int j = threadIdx.x;
-  const int i0 = blockIdx.x * 256;
-  float s = 0.f;
-  for (int i = i0; i < min(n, i0 + 256); ++i) s += float(dZ[size_t(i) * WIDTH + j]);
-  atomicAdd(&gb[j], s);
 }
 
 __global__ void k_zero(float* p, int n)
@@ -423,23 +435,19 @@ rg_status backward(rg_mlp* p, int n, double beta)
   const dim3 blk(256);
   hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), blk, 0, st, p->g, int(NPARAM));
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(1024), 0, st, n, p->f, p->U, beta, p->df, p->loss);
-  const int rb = (n + 255) / 256;
-  hipLaunchKernelGGL(k_out_backward, dim3(rb), dim3(WIDTH), 0, st, n, p->H3, p->Z3, p->wb + OFF_W4, p->df,
-                     p->g + OFF_W4, p->g + OFF_B4, p->dZa);
-  // layer 3: dW3 = H2^T dZ3, db3, dZ2 = (dZ3 W3^T) * SiLU'(Z2)
+  hipLaunchKernelGGL(k_out_backward, dim3((n + OB_ROWS - 1) / OB_ROWS), dim3(WIDTH), 0, st, n, p->H3, p->Z3,
+                     p->wb + OFF_W4, p->df, p->g + OFF_W4, p->g + OFF_B4, p->dZa, p->g + OFF_B3);
+  // layer 3: dW3 = H2^T dZ3; dZ2 = (dZ3 W3^T) * SiLU'(Z2) with db2 = column sums of dZ2
   rg_status ws;
   if ((ws = weightGrad(p->ctx, WIDTH, n, p->H2, p->dZa, p->g + OFF_W3)) != RG_OK) return ws;
-  hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZa, p->g + OFF_B3);
   hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
-                     p->dZa, WIDTH, p->wb + OFF_W3, WIDTH, nullptr, p->Z2, nullptr, p->dZb);
-  // layer 2
+                     p->dZa, WIDTH, p->wb + OFF_W3, WIDTH, nullptr, p->Z2, p->g + OFF_B2, p->dZb);
+  // layer 2: dW2 = H1^T dZ2; dZ1 = (dZ2 W2^T) * SiLU'(Z1) with db1
   if ((ws = weightGrad(p->ctx, WIDTH, n, p->H1, p->dZb, p->g + OFF_W2)) != RG_OK) return ws;
-  hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZb, p->g + OFF_B2);
   hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
-                     p->dZb, WIDTH, p->wb + OFF_W2, WIDTH, nullptr, p->Z1, nullptr, p->dZa);
-  // layer 1: dW1 = X^T dZ1, db1
+                     p->dZb, WIDTH, p->wb + OFF_W2, WIDTH, nullptr, p->Z1, p->g + OFF_B1, p->dZa);
+  // layer 1: dW1 = X^T dZ1
   if ((ws = weightGrad(p->ctx, NIN, n, p->X, p->dZa, p->g + OFF_W1)) != RG_OK) return ws;
-  hipLaunchKernelGGL(k_colsum, dim3(rb), dim3(WIDTH), 0, st, n, p->dZa, p->g + OFF_B1);
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
 }
