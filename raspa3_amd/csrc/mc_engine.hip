@@ -111,21 +111,39 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
       if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.fwq[a] * q);
     }
   }
-  for (int a = lane; a < nads; a += 64)
+  // adsorbate atoms, four per lane and pass so that one memory latency covers their loads
+  constexpr int U = 4;
+  for (int a0 = 0; a0 < nads; a0 += 64 * U)
   {
-    const int m = S.amol[a];
-    const int t = S.at[a];
-    if (m == own || m == skip || t < 0) continue;
-    double dx = S.ax[a] - px, dy = S.ay[a] - py, dz = S.az[a] - pz;
-    rgd::pbc(S.box, dx, dy, dz);
-    const double rr = dx * dx + dy * dy + dz * dz;
-    if (rr < S.rcv2Mol)
+    double ax[U], ay[U], az[U], aq[U];
+    int am[U], at[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
     {
-      const double v = rgd::vdw0(S.pairs[t * S.ntypes + type], 1.0, rr);
-      o = o || v > S.overlap;
-      e += v;
+      const int a = a0 + u * 64 + lane;
+      const bool in = a < nads;
+      am[u] = in ? S.amol[a] : -3;
+      at[u] = in ? S.at[a] : -1;
+      ax[u] = in ? S.ax[a] : 0.0;
+      ay[u] = in ? S.ay[a] : 0.0;
+      az[u] = in ? S.az[a] : 0.0;
+      aq[u] = in ? S.aq[a] : 0.0;
     }
-    if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.aq[a] * q);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      if (am[u] == own || am[u] == skip || at[u] < 0) continue;
+      double dx = ax[u] - px, dy = ay[u] - py, dz = az[u] - pz;
+      rgd::pbc(S.box, dx, dy, dz);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < S.rcv2Mol)
+      {
+        const double v = rgd::vdw0(S.pairs[at[u] * S.ntypes + type], 1.0, rr);
+        o = o || v > S.overlap;
+        e += v;
+      }
+      if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * aq[u] * q);
+    }
   }
   e = rgd::waveSum(e);
   if (withGrid && gi >= 0)
@@ -245,35 +263,67 @@ __device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, doub
     }
   }
   waveSyncLds();
+  // four k-vectors per lane and pass: their loads are issued together, so one memory latency covers four
+  constexpr int U = 4;
   double de = 0.0;
-  for (int k = lane; k < S.nk; k += 64)
+  for (int k0 = 0; k0 < S.nk; k0 += 64 * U)
   {
-    const int code = S.kcode[k];
-    const int kx = code & 0xff, ky = ((code >> 8) & 0xff) - 128, kz = ((code >> 16) & 0xff) - 128;
-    double2 d = make_double2(0.0, 0.0);
-    for (int a = 0; a < na; ++a)
+    int code[U];
+    double2 sk[U];
+    double fk[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
     {
-      const double2* row = tab + a * stride;
-      const double2 ex = row[kx];
-      double2 ey = row[(EW_MAXK + 1) + (ky < 0 ? -ky : ky)];
-      if (ky < 0) ey.y = -ey.y;
-      double2 ez = row[2 * (EW_MAXK + 1) + (kz < 0 ? -kz : kz)];
-      if (kz < 0) ez.y = -ez.y;
-      const double2 e = cmul(cmul(ex, ey), ez);
-      d.x += at[4 * a + 3] * e.x;
-      d.y += at[4 * a + 3] * e.y;
+      const int k = k0 + u * 64 + lane;
+      const bool in = k < S.nk;
+      code[u] = in ? S.kcode[k] : 0;
+      sk[u] = in ? S.Sk[k] : make_double2(0.0, 0.0);
+      fk[u] = in ? S.fk[k] : 0.0;
     }
-    const double2 s = S.Sk[k];
-    const double2 t = make_double2(s.x + d.x, s.y + d.y);
-    de += S.fk[k] * ((t.x * t.x + t.y * t.y) - (s.x * s.x + s.y * s.y));
-    if (writeTrial) S.St[k] = t;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const int k = k0 + u * 64 + lane;
+      const int kx = code[u] & 0xff, ky = ((code[u] >> 8) & 0xff) - 128, kz = ((code[u] >> 16) & 0xff) - 128;
+      double2 d = make_double2(0.0, 0.0);
+      for (int a = 0; a < na; ++a)
+      {
+        const double2* row = tab + a * stride;
+        const double2 ex = row[kx];
+        double2 ey = row[(EW_MAXK + 1) + (ky < 0 ? -ky : ky)];
+        if (ky < 0) ey.y = -ey.y;
+        double2 ez = row[2 * (EW_MAXK + 1) + (kz < 0 ? -kz : kz)];
+        if (kz < 0) ez.y = -ez.y;
+        const double2 e = cmul(cmul(ex, ey), ez);
+        d.x += at[4 * a + 3] * e.x;
+        d.y += at[4 * a + 3] * e.y;
+      }
+      const double2 t = make_double2(sk[u].x + d.x, sk[u].y + d.y);
+      de += fk[u] * ((t.x * t.x + t.y * t.y) - (sk[u].x * sk[u].x + sk[u].y * sk[u].y));
+      if (writeTrial && k < S.nk) S.St[k] = t;
+    }
   }
   return rgd::waveSum(de);
 }
-
 __device__ void waveAcceptEwald(const SysDev& S, int lane)
 {
-  for (int k = lane; k < S.nk; k += 64) S.Sk[k] = S.St[k];
+  constexpr int U = 4;
+  for (int k0 = 0; k0 < S.nk; k0 += 64 * U)
+  {
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const int k = k0 + u * 64 + lane;
+      if (k < S.nk) v[u] = S.St[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const int k = k0 + u * 64 + lane;
+      if (k < S.nk) S.Sk[k] = v[u];
+    }
+  }
 }
 
 
@@ -613,15 +663,32 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
       W.pos[3 * (cm.na + lane) + 2] = nz;
     }
     waveSyncLds();
-    double Uold = 0.0, Unew = 0.0;
-    bool ovNew = false;
+    // grid parts of the old (lanes 0 .. na-1) and new (lanes na .. 2na-1) sites, interpolated concurrently
+    double gl = 0.0;
+    bool gov = false;
+    if (lane < 2 * cm.na && S.gridOfType)
+    {
+      const int a = lane < cm.na ? lane : lane - cm.na;
+      const int gi = S.gridOfType[cm.type[a]];
+      if (gi >= 0)
+      {
+        const double* p = W.pos + 3 * lane;
+        gl = rgd::interpolate(S.grids[gi], p[0], p[1], p[2]);
+        gov = gl > S.overlap;
+        if (S.useCharge && S.coulGrid >= 0) gl += cm.q[a] * rgd::interpolate(S.grids[S.coulGrid], p[0], p[1], p[2]);
+      }
+    }
+    const bool newLane = lane >= cm.na && lane < 2 * cm.na;
+    double Uold = rgd::waveSum(lane < cm.na ? gl : 0.0);
+    double Unew = rgd::waveSum(newLane ? gl : 0.0);
+    bool ovNew = __ballot(newLane && gov) != 0ull;
     for (int a = 0; a < cm.na; ++a)
     {
       bool o;
       Uold += siteExternal(S, L.nads, W.pos[3 * a], W.pos[3 * a + 1], W.pos[3 * a + 2], cm.type[a], cm.q[a], m, -2,
-                           lane, o);
+                           lane, o, false);
       Unew += siteExternal(S, L.nads, W.pos[3 * (cm.na + a)], W.pos[3 * (cm.na + a) + 1], W.pos[3 * (cm.na + a) + 2],
-                           cm.type[a], cm.q[a], m, -2, lane, o);
+                           cm.type[a], cm.q[a], m, -2, lane, o, false);
       ovNew = ovNew || o;
     }
     if (ovNew) return;
