@@ -300,7 +300,7 @@ def _mc_lib():
         L.orc_mc_adsorbates.restype = C.c_int
         L.orc_widom.argtypes = [vp, C.c_int, C.c_longlong, C.c_int, dp, dp, dp]
         L.orc_mc_molecule_weights.restype = C.c_int
-        L.orc_mc_molecule_weights.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_ulonglong, dp]
+        L.orc_mc_molecule_weights.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_ulonglong, dp, dp]
         L._mc_ready = True
     return L
 
@@ -389,12 +389,16 @@ class MC:
         _mc_lib().orc_widom(self.h, component, first, count, _d(lw), _d(u), _d(ew))
         return lw, u, ew
 
-    def molecule_weights(self, component, frac, n_orient, seed):
+    def molecule_weights(self, component, frac, n_orient, seed, mean_energy=False):
+        """log of the orientation-averaged Boltzmann factor per point (and the Boltzmann-weighted mean energy over
+        the orientations when mean_energy)."""
         frac = _f(frac).reshape(-1, 3)
         out = np.zeros(len(frac))
-        st = _mc_lib().orc_mc_molecule_weights(self.h, component, len(frac), _d(frac), n_orient, seed, _d(out))
+        mu = np.zeros(len(frac)) if mean_energy else None
+        st = _mc_lib().orc_mc_molecule_weights(self.h, component, len(frac), _d(frac), n_orient, seed, _d(out),
+                                               _d(mu) if mu is not None else None)
         assert st == 0, "every atom type of the component needs a grid"
-        return out
+        return (out, mu) if mean_energy else out
 
     def __del__(self):
         try:

@@ -89,3 +89,42 @@ def test_gcmc_biased_first_bead_proposal_keeps_henry_law():
     b = mc.state()["blocks"]
     mean = (b[:, 2] / b[:, 0]).mean()
     assert abs(mean / target - 1.0) < 0.1, mean
+
+
+def test_rotation_moves_sample_boltzmann_orientations():
+    """Pins the rigid-body rotation move (and translation) of the oracle for a multi-site molecule.  An uncharged
+    three-site CO2 in ITQ-29 at a pressure where <N> ~ 0.3: with N in {0, 1}, <U>/<N> from the block sums is the
+    energy of one molecule averaged over the Boltzmann distribution of its position and orientation.  Rotation moves
+    make 60 % of the moves, so an acceptance rule that mis-weights orientations moves this average; the reference
+    value is a quadrature over 60,000 uniform positions x 16 orientations (energy-weighted Boltzmann average)."""
+    spec = w.forcefield_zeolite()
+    spec.use_charge = False
+    fw, unit = w.framework_itq29(spec, 3)
+    dims = w.grid_dims(unit, 0.25)[0]
+    pairs = oracle.pairs_from_spec(spec)
+    g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
+                              spec.alpha, nthreads=8) for t in (spec.index("C_co2"), spec.index("O_co2"))}
+    comp = w.co2_reference(spec)
+    comp["charge"] = [0.0, 0.0, 0.0]
+    e2k = oracle.units()["energy_to_kelvin"]
+    beta = 1.0 / (oracle.units()["kB"] * 300.0)
+    q = oracle.MC(spec, fw, [comp], w.mc_params(), kmax=(0, 0, 0), grids=(g, None, dims, unit))
+    frac = np.random.default_rng(1).random((60000, 3))
+    lw, mu = q.molecule_weights(0, frac, 16, 5, mean_energy=True)
+    W = np.exp(lw)
+    u_ref = (W * mu).sum() / W.sum() * e2k
+    err_ref = np.std([(W[i::10] * mu[i::10]).sum() / W[i::10].sum() for i in range(10)], ddof=1) / np.sqrt(10) * e2k
+    vol = abs(np.linalg.det(fw.cell))
+    pconv = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+    P = 0.3 / (beta * vol * W.mean()) * pconv
+    mc = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=P, p_translation=0.2, p_rotation=0.6,
+                                                 production_cycles=40000, seed=3),
+                   kmax=(0, 0, 0), grids=(g, None, dims, unit))
+    mc.run(200)
+    mc.run(40000, sample=True)
+    s = mc.state()
+    b = s["blocks"]
+    per = b[:, 1] / b[:, 2] * e2k
+    u_mc, err_mc = per.mean(), per.std(ddof=1) / np.sqrt(len(per))
+    assert s["accepted"][3] > 50000
+    assert abs(u_mc - u_ref) < 4.0 * np.hypot(err_mc, err_ref) + 10.0, (u_mc, err_mc, u_ref, err_ref)
