@@ -299,10 +299,16 @@ def _mc_lib():
         L.orc_mc_adsorbates.argtypes = [vp, dp, ip, ip]
         L.orc_mc_adsorbates.restype = C.c_int
         L.orc_widom.argtypes = [vp, C.c_int, C.c_longlong, C.c_int, dp, dp, dp]
+        L.orc_mc_grow.restype = C.c_int
+        L.orc_mc_grow.argtypes = [vp, C.c_int, C.c_longlong, dp, dp, dp]
         L.orc_mc_molecule_weights.restype = C.c_int
         L.orc_mc_molecule_weights.argtypes = [vp, C.c_int, C.c_int, dp, C.c_int, C.c_ulonglong, dp, dp]
         L._mc_ready = True
     return L
+
+
+SLOTS = 8          # atom slots per molecule in the Monte Carlo adsorbate arrays
+COMP_INT, COMP_DBL = 11, 41  # component record sizes of orc_mc_create
 
 
 class MC:
@@ -337,16 +343,24 @@ class MC:
                 self._keep.append(cg)
             gdims = _n(dims)
             self._keep.append(gdims)
-        ci = np.zeros(6 * len(comps), np.int32)
-        cd = np.zeros(17 * len(comps))
+        ci = np.zeros(COMP_INT * len(comps), np.int32)
+        cd = np.zeros(COMP_DBL * len(comps))
         for c, d in enumerate(comps):
             na = d["n_atoms"]
-            ci[6 * c] = na
-            ci[6 * c + 1] = d["first_bead"]
-            ci[6 * c + 2:6 * c + 2 + na] = d["type"]
-            cd[17 * c:17 * c + 3 * na] = np.asarray(d["position"], float).reshape(-1)
-            cd[17 * c + 12:17 * c + 12 + na] = d["charge"]
-            cd[17 * c + 16] = d["mol_fraction"]
+            i0, d0 = COMP_INT * c, COMP_DBL * c
+            ci[i0] = na
+            ci[i0 + 1] = d["first_bead"]
+            ci[i0 + 2] = int(d.get("flexible", False))
+            ci[i0 + 3:i0 + 3 + na] = d["type"]
+            cd[d0:d0 + 3 * na] = np.asarray(d["position"], float).reshape(-1)
+            cd[d0 + 24:d0 + 24 + na] = d["charge"]
+            cd[d0 + 32] = d["mol_fraction"]
+            if d.get("flexible", False):
+                cd[d0 + 33] = d["bond_length"]
+                cd[d0 + 34] = d["bend_theta0_deg"]
+                cd[d0 + 35] = d["bend_k_K"]
+                cd[d0 + 36:d0 + 40] = d["torsion_K"]
+                cd[d0 + 40] = d.get("log_ideal_rosenbluth", 0.0)
         dpar = _f([params["temperature"], params["pressure_pa"], params["p_translation"], params["p_rotation"],
                    params["max_translation"], params["max_rotation"]])
         ipar = (C.c_longlong * 6)(params["k_first"], params["k_orient"], params["min_steps_per_cycle"],
@@ -382,7 +396,14 @@ class MC:
         t = np.zeros(capacity, np.int32)
         c = np.zeros(capacity, np.int32)
         n = _mc_lib().orc_mc_adsorbates(self.h, _d(pos), _i(t), _i(c))
-        return pos[:n], t[:n], c[:n // 4]
+        return pos[:n], t[:n], c[:n // SLOTS]
+
+    def grow(self, component, step, n_atoms):
+        """One growth of a molecule at move 'step' (no acceptance): positions, log W, energy (None if it failed)."""
+        pos = np.zeros((n_atoms, 3))
+        lw, u = C.c_double(), C.c_double()
+        ok = _mc_lib().orc_mc_grow(self.h, component, step, _d(pos), C.byref(lw), C.byref(u))
+        return (pos, lw.value, u.value) if ok else None
 
     def widom(self, component, first, count):
         lw, u, ew = np.zeros(count), np.zeros(count), np.zeros(count)

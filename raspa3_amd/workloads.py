@@ -260,6 +260,23 @@ def co2_reference(spec, mol_fraction=1.0):
                         spec.charge[spec.index("O_co2")]], mol_fraction=mol_fraction)
 
 
+def forcefield_hexane_zeolite():
+    """The zeolite force field plus TraPPE united-atom CH3 (98 K, 3.75 A) and CH2 (46 K, 3.95 A)."""
+    return forcefield_zeolite([("CH3", 98.0, 3.75, 0.0), ("CH2", 46.0, 3.95, 0.0)])
+
+
+def hexane_component(spec, log_ideal_rosenbluth=0.0, mol_fraction=1.0):
+    """United-atom n-hexane (examples/non_basic/8_mc_adsorption_c6_isomers_in_mfi/hexane.json): CH3-(CH2)4-CH3, bonds
+    1.54 A, bend 114 deg with k = 62500 K/rad^2, torsion c0 + c1 (1 + cos p) + c2 (1 - cos 2p) + c3 (1 + cos 3p) with
+    [0, 355.03, -68.19, 791.32] K (trans at 180 deg), LJ between beads more than three bonds apart.  Grown from
+    bead 0.  log_ideal_rosenbluth: log of the ideal-gas Rosenbluth weight of the growth scheme."""
+    t3, t2 = spec.index("CH3"), spec.index("CH2")
+    return dict(n_atoms=6, first_bead=0, position=[[0.0, 0.0, 1.54 * i] for i in range(6)],
+                type=[t3, t2, t2, t2, t2, t3], charge=[0.0] * 6, mol_fraction=mol_fraction, flexible=True,
+                bond_length=1.54, bend_theta0_deg=114.0, bend_k_K=62500.0, torsion_K=[0.0, 355.03, -68.19, 791.32],
+                log_ideal_rosenbluth=log_ideal_rosenbluth)
+
+
 def methane_component(spec):
     return dict(n_atoms=1, first_bead=0, position=[[0, 0, 0]], type=[spec.index("CH4")], charge=[0.0],
                 mol_fraction=1.0)

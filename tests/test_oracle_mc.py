@@ -128,3 +128,73 @@ def test_rotation_moves_sample_boltzmann_orientations():
     u_mc, err_mc = per.mean(), per.std(ddof=1) / np.sqrt(len(per))
     assert s["accepted"][3] > 50000
     assert abs(u_mc - u_ref) < 4.0 * np.hypot(err_mc, err_ref) + 10.0, (u_mc, err_mc, u_ref, err_ref)
+
+
+def empty_box(L=40.0):
+    return w.Framework("empty", np.diag([L] * 3), True, np.zeros((0, 3)), np.zeros(0, np.int32), np.zeros(0))
+
+
+def test_chain_growth_samples_bonded_boltzmann():
+    """Flexible-chain growth (united-atom n-hexane) with the non-bonded energies switched off: every selected bend
+    angle follows p(theta) ~ sin(theta) exp(-U_bend / kT) and every torsion p(phi) ~ exp(-U_tors / kT), computed here
+    from the grown coordinates with independent angle and dihedral formulas and compared with numerical integrals
+    of the closed forms (mean and spread of theta, trans fraction and <cos phi>)."""
+    spec = w.forcefield_hexane_zeolite()
+    ideal = w._ff([(n, 0.0, s, 0.0, f) for n, s, f in zip(spec.names, spec.sigma, spec.is_framework)],
+                  use_charge=False)
+    comp = w.hexane_component(ideal)
+    mc = oracle.MC(ideal, empty_box(), [comp], w.mc_params(seed=4), kmax=(0, 0, 0))
+    th, ph = [], []
+    for s in range(12000):
+        p = mc.grow(0, s, 6)[0]
+        for i in range(2, 6):
+            u, v = p[i - 2] - p[i - 1], p[i] - p[i - 1]
+            th.append(np.arccos(u @ v / np.linalg.norm(u) / np.linalg.norm(v)))
+            assert abs(np.linalg.norm(v) - 1.54) < 1e-12
+        for i in range(3, 6):
+            b1, b2, b3 = p[i - 2] - p[i - 3], p[i - 1] - p[i - 2], p[i] - p[i - 1]
+            n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+            ph.append(np.arctan2(np.cross(n1, b2 / np.linalg.norm(b2)) @ n2, n1 @ n2))
+    th, ph = np.array(th), np.array(ph)
+    kT = 300.0
+    g = np.linspace(0.0, np.pi, 200001)
+    pb = np.sin(g) * np.exp(-0.5 * 62500.0 * (g - np.radians(114.0)) ** 2 / kT)
+    mean_b = (g * pb).sum() / pb.sum()
+    sd_b = np.sqrt(((g - mean_b) ** 2 * pb).sum() / pb.sum())
+    assert abs(th.mean() - mean_b) < 4 * sd_b / np.sqrt(len(th))
+    assert abs(th.std() / sd_b - 1.0) < 0.02
+    f = np.linspace(-np.pi, np.pi, 200001)
+    pt = np.exp(-(355.03 * (1 + np.cos(f)) - 68.19 * (1 - np.cos(2 * f)) + 791.32 * (1 + np.cos(3 * f))) / kT)
+    trans = pt[np.abs(f) > 2 * np.pi / 3].sum() / pt.sum()
+    cos_mean = (np.cos(f) * pt).sum() / pt.sum()
+    n = len(ph)
+    assert abs((np.abs(ph) > 2 * np.pi / 3).mean() - trans) < 4 * np.sqrt(trans * (1 - trans) / n)
+    assert abs(np.cos(ph).mean() - cos_mean) < 4 * np.sqrt((np.cos(f) ** 2 * pt).sum() / pt.sum() / n)
+
+
+def test_gcmc_hexane_follows_henry_law():
+    """Chain CBMC acceptance: n-hexane in ITQ-29 at low pressure loads <N> = beta P V <W> / W_IG, with <W> from
+    Widom insertions in the framework and W_IG from Widom insertions into an empty box (the intra-molecular
+    LJ of the 1-5 and 1-6 pairs makes W_IG ~ 1.35, so leaving it out would miss by 35 %)."""
+    spec = w.forcefield_hexane_zeolite()
+    spec.use_charge = False
+    e = oracle.MC(spec, empty_box(), [w.hexane_component(spec)], w.mc_params(seed=4), kmax=(0, 0, 0))
+    wig = np.exp(e.widom(0, 0, 40000)[0]).mean()
+    comp = w.hexane_component(spec, np.log(wig))
+    fw, unit = w.framework_itq29(spec, 3)
+    dims = w.grid_dims(unit, 0.25)[0]
+    pairs = oracle.pairs_from_spec(spec)
+    g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
+                              spec.alpha, nthreads=8) for t in (spec.index("CH3"), spec.index("CH2"))}
+    grids = (g, None, dims, unit)
+    W = np.exp(oracle.MC(spec, fw, [comp], w.mc_params(seed=7), kmax=(0, 0, 0), grids=grids)
+               .widom(0, 0, 20000)[0]).mean() / wig
+    beta = 1.0 / (oracle.units()["kB"] * 300.0)
+    pconv = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+    P = 0.25 / (beta * abs(np.linalg.det(fw.cell)) * W) * pconv
+    mc = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=P, p_translation=0.3, p_rotation=0.2,
+                                                 production_cycles=8000, seed=21), kmax=(0, 0, 0), grids=grids)
+    mc.run(100)
+    mc.run(8000, sample=True)
+    b = mc.state()["blocks"]
+    assert abs((b[:, 2] / b[:, 0]).mean() / 0.25 - 1.0) < 0.1
