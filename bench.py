@@ -279,7 +279,7 @@ def bench_gcmc(ctx, args):
 
     def run(label, seed, proposal=None):
         systems = [ra.System(ctx, ff, fw.cell, True, framework=fb, grids=grids, coulomb_grid=coul,
-                             kmax=tuple(kmax), capacity=4 * 256) for _ in range(nsys)]
+                             kmax=tuple(kmax), capacity=8 * 256) for _ in range(nsys)]
         params = w.mc_params(temperature=T, pressure_pa=P, seed=seed, max_molecules=256,
                              production_cycles=args.gcmc_production)
         mc = ra.Mc(ctx, systems, [comp], params)

@@ -204,16 +204,28 @@ rg_status rg_ewald_energy(rg_ctx* ctx, const rg_system* sys, double* out);
  * src/raspakit/cbmc/multiple_first_bead.cpp; orientations of the rigid body src/raspakit/cbmc/operators.cpp:958-968).
  * ------------------------------------------------------------------------------------------- */
 
-/* A rigid component (Component, src/raspakit/component.ixx): atoms in a reference geometry; the first bead is the
- * atom grown first (Component::startingBead). */
+/* A component (Component, src/raspakit/component.ixx).  Rigid (flexible = 0): up to 4 atoms in a reference geometry;
+ * the first bead is the atom grown first (Component::startingBead).  Flexible linear chain (flexible = 1, uncharged,
+ * up to 8 united-atom beads bonded in index order, first_bead = 0): bonds of fixed length, harmonic bend
+ * U = k (theta - theta0)^2 / 2, torsion U = c0 + c1 (1 + cos p) + c2 (1 - cos 2p) + c3 (1 + cos 3p) (trans at 180 deg),
+ * Lennard-Jones between beads more than three bonds apart (src/raspakit/cbmc/flexible_molecules/insertion.cpp).
+ * Bend and torsion are sampled exactly from their Boltzmann factors, so growth weights carry the external and
+ * intra-molecular non-bonded energies; log_ideal_rosenbluth is log W_IG of that growth (Widom insertions into an
+ * empty box give it), divided out in the acceptance rules. */
 typedef struct rg_component_desc
 {
-  int n_atoms;          /* 1..4 */
+  int n_atoms;          /* 1..4 rigid, 2..8 flexible */
   int first_bead;
-  double position[12];  /* n_atoms * 3, reference geometry (A) */
-  int type[4];          /* pseudo-atom indices */
-  double charge[4];
+  double position[24];  /* n_atoms * 3, reference geometry (A); unused for flexible chains */
+  int type[8];          /* pseudo-atom indices */
+  double charge[8];
   double mol_fraction;  /* Component::molFraction */
+  int flexible;
+  double bond_length;       /* A */
+  double bend_theta0_deg;
+  double bend_k_K;          /* K / rad^2 */
+  double torsion_K[4];      /* c0 .. c3 in K */
+  double log_ideal_rosenbluth;
 } rg_component_desc;
 
 typedef struct rg_mc_params
@@ -263,7 +275,7 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
  * On return the systems' adsorbate arrays, structure factors, adsorbate count and net charge are the final state. */
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);
 rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
-/* Adsorbate atoms of a system (4 slots per molecule; unused slots have type -1) and the component of each molecule. */
+/* Adsorbate atoms of a system (8 slots per molecule; unused slots have type -1) and the component of each molecule. */
 rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacity, double* pos, int* type,
                            int* component, int* n_out);
 /* Training data of the learned proposal: the orientation-averaged Boltzmann factor of a rigid molecule with its

@@ -288,7 +288,8 @@ def build_grids(ctx, ff, fw, specs, sync=True):
 
 
 def component_desc(d):
-    """rg_component_desc from a dict(n_atoms, first_bead, position (n,3), type, charge, mol_fraction)."""
+    """rg_component_desc from a dict(n_atoms, first_bead, position (n,3), type, charge, mol_fraction[, flexible,
+    bond_length, bend_theta0_deg, bend_k_K, torsion_K, log_ideal_rosenbluth])."""
     c = L.ComponentDesc()
     c.n_atoms = d["n_atoms"]
     c.first_bead = d["first_bead"]
@@ -299,6 +300,14 @@ def component_desc(d):
         c.type[a] = int(d["type"][a])
         c.charge[a] = float(d["charge"][a])
     c.mol_fraction = d.get("mol_fraction", 1.0)
+    if d.get("flexible", False):
+        c.flexible = 1
+        c.bond_length = d["bond_length"]
+        c.bend_theta0_deg = d["bend_theta0_deg"]
+        c.bend_k_K = d["bend_k_K"]
+        for k in range(4):
+            c.torsion_K[k] = d["torsion_K"][k]
+        c.log_ideal_rosenbluth = d.get("log_ideal_rosenbluth", 0.0)
     return c
 
 
@@ -380,7 +389,7 @@ class Mc(_Handle):
         n = C.c_int()
         L.check(lib().rg_mc_adsorbates(self.ctx.h, self.h, system, capacity, L.d(pos), L.i(t), L.i(c), C.byref(n)),
                 self.ctx.h, "rg_mc_adsorbates")
-        return pos[:n.value], t[:n.value], c[:n.value // 4]
+        return pos[:n.value], t[:n.value], c[:n.value // L.SLOTS]
 
     def total_energy(self):
         out = np.zeros(len(self.systems))

@@ -101,9 +101,14 @@ def check(status, ctx=None, what=""):
         raise RGError(f"{what}: status {status}: {msg}")
 
 
+SLOTS = 8  # atom slots per molecule in the Monte Carlo adsorbate arrays
+
+
 class ComponentDesc(C.Structure):
-    _fields_ = [("n_atoms", C.c_int), ("first_bead", C.c_int), ("position", C.c_double * 12),
-                ("type", C.c_int * 4), ("charge", C.c_double * 4), ("mol_fraction", C.c_double)]
+    _fields_ = [("n_atoms", C.c_int), ("first_bead", C.c_int), ("position", C.c_double * 24),
+                ("type", C.c_int * 8), ("charge", C.c_double * 8), ("mol_fraction", C.c_double),
+                ("flexible", C.c_int), ("bond_length", C.c_double), ("bend_theta0_deg", C.c_double),
+                ("bend_k_K", C.c_double), ("torsion_K", C.c_double * 4), ("log_ideal_rosenbluth", C.c_double)]
 
 
 class McParams(C.Structure):

@@ -276,7 +276,7 @@ __device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, doub
     {
       const int k = k0 + u * 64 + lane;
       const bool in = k < S.nk;
-      code[u] = in ? S.kcode[k] : 0;
+      code[u] = in ? S.kcode[k] : (128 << 8) | (128 << 16);  // padding decodes to k = (0, 0, 0)
       sk[u] = in ? S.Sk[k] : make_double2(0.0, 0.0);
       fk[u] = in ? S.fk[k] : 0.0;
     }
@@ -298,9 +298,10 @@ __device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, doub
         d.x += at[4 * a + 3] * e.x;
         d.y += at[4 * a + 3] * e.y;
       }
+      if (k >= S.nk) continue;
       const double2 t = make_double2(sk[u].x + d.x, sk[u].y + d.y);
       de += fk[u] * ((t.x * t.x + t.y * t.y) - (sk[u].x * sk[u].x + sk[u].y * sk[u].y));
-      if (writeTrial && k < S.nk) S.St[k] = t;
+      if (writeTrial) S.St[k] = t;
     }
   }
   return rgd::waveSum(de);
@@ -372,17 +373,16 @@ __device__ double proposalLogBias(const McSys& Y, const SysDev& S, const double 
   return -P.logNvoxP[idx[0] + P.n[0] * (idx[1] + P.n[1] * idx[2])];
 }
 
-// Grows a molecule of component c: K first-bead trials, then (for more than one atom) K orientation trials about
-// the selected first bead.  On success writes the atom positions into W.mol and returns true with the log Rosenbluth
-// weight and the external energy of the grown molecule.  'retrace' >= 0 retraces that existing molecule instead
-// (trial 0 = its current first bead and orientation; its own atoms are excluded).
-__device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+// First-bead step of a growth: kFirst trials (uniform or from the proposal; the existing first bead is trial 0 of a
+// retrace), Rosenbluth selection.  On success every lane holds the selected position, its energy and log weight,
+// and the second selection uniform of the move.
+__device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
+                              unsigned long long step, int retrace, int lane, WaveScratch& W, double& fx, double& fy,
+                              double& fz, double& Ufirst, double& logW1, double& uSel2)
 {
-  const int K1 = R.kFirst, K2 = R.kOrient;
+  const int K1 = R.kFirst;
   const int own = retrace >= 0 ? retrace : -2;
   const int fb = c.first;
-  // --- first bead
   double p[3] = {0.0, 0.0, 0.0};
   double logBias = 0.0;
   if (lane < K1)
@@ -406,16 +406,32 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
   bool ov1;
   const int fbIdx[1] = {fb};
   trialSetEnergies(S, nads, c, fbIdx, 1, K1, W.pos, own, -2, lane, U1, ov1);
-  double uSel1, uSel2;
+  double uSel1;
   uni(R, sys, step, SLOT_SELECT, uSel1, uSel2);
   int sel1;
-  double logW1;
   waveRosenbluth(U1, ov1, logBias, K1, R.beta, uSel1, lane, sel1, logW1);
   if (retrace >= 0) sel1 = 0;  // the retrace keeps the existing molecule
   if (sel1 < 0 || logW1 < R.logMinW) return false;
-  const double fx = __shfl(p[0], sel1, 64), fy = __shfl(p[1], sel1, 64), fz = __shfl(p[2], sel1, 64);
-  const double Ufirst = __shfl(U1, sel1, 64);
+  fx = __shfl(p[0], sel1, 64);
+  fy = __shfl(p[1], sel1, 64);
+  fz = __shfl(p[2], sel1, 64);
+  Ufirst = __shfl(U1, sel1, 64);
   if (retrace >= 0 && __shfl(ov1 ? 1 : 0, 0, 64)) return false;
+  return true;
+}
+
+// Grows a molecule of component c: K first-bead trials, then (for more than one atom) K orientation trials about
+// the selected first bead.  On success writes the atom positions into W.mol and returns true with the log Rosenbluth
+// weight and the external energy of the grown molecule.  'retrace' >= 0 retraces that existing molecule instead
+// (trial 0 = its current first bead and orientation; its own atoms are excluded).
+__device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
+                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+{
+  const int K2 = R.kOrient;
+  const int own = retrace >= 0 ? retrace : -2;
+  const int fb = c.first;
+  double fx, fy, fz, Ufirst, logW1, uSel2;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, fx, fy, fz, Ufirst, logW1, uSel2)) return false;
   if (c.na == 1)
   {
     if (lane == 0)
@@ -486,6 +502,254 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
   return true;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// flexible linear chains (united-atom alkanes): bond geometry sampled exactly from the bonded Boltzmann factors
+// (Frenkel & Smit, Understanding Molecular Simulation, 2nd ed., sec. 13.3), Rosenbluth weights over the external
+// and intra-molecular non-bonded energies.  Lane j builds trial j of a bead; its rejection samplers draw from the
+// Philox slots SLOT_CHAIN + ((bead * 64 + trial) * 4 + kind) * MC_MAXREJ + attempt, the bead's selection from
+// SLOT_CHAIN - 16 + bead.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr uint32_t SLOT_CHAIN = 1024;
+
+__device__ __forceinline__ void uniChain(const McRun& R, int sys, unsigned long long step, int bead, int trial,
+                                         int kind, int attempt, double& a, double& b)
+{
+  uni(R, sys, step, SLOT_CHAIN + uint32_t(((bead * 64 + trial) * 4 + kind) * MC_MAXREJ + attempt), a, b);
+}
+
+__device__ __forceinline__ double chainTorsionEnergy(const Comp& c, double phi)
+{
+  return c.tors[0] + c.tors[1] * (1.0 + cos(phi)) + c.tors[2] * (1.0 - cos(2.0 * phi)) +
+         c.tors[3] * (1.0 + cos(3.0 * phi));
+}
+
+// p(theta) ~ sin(theta) exp(-beta k (theta - theta0)^2 / 2): Gaussian proposal (Box-Muller), accepted with sin(theta)
+__device__ double sampleBend(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial)
+{
+  if (c.kbend <= 0.0)
+  {
+    double u, v;
+    uniChain(R, sys, step, bead, trial, 1, 0, u, v);
+    return acos(1.0 - 2.0 * u);
+  }
+  const double sd = 1.0 / sqrt(R.beta * c.kbend);
+  for (int t = 0; t < MC_MAXREJ; ++t)
+  {
+    double g1, g2, a, b;
+    uniChain(R, sys, step, bead, trial, 1, t, g1, g2);
+    uniChain(R, sys, step, bead, trial, 2, t, a, b);
+    const double th = c.theta0 + sd * sqrt(-2.0 * log(1.0 - g1)) * cos(2.0 * M_PI * g2);
+    if (th > 0.0 && th < M_PI && a < sin(th)) return th;
+  }
+  return c.theta0;
+}
+
+// p(phi) ~ exp(-beta U_tors(phi)) on [-pi, pi): uniform proposal accepted with exp(-beta (U - U_min))
+__device__ double sampleTorsion(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial)
+{
+  for (int t = 0; t < MC_MAXREJ; ++t)
+  {
+    double u, a;
+    uniChain(R, sys, step, bead, trial, 3, t, u, a);
+    const double phi = 2.0 * M_PI * u - M_PI;
+    if (a < exp(-R.beta * (chainTorsionEnergy(c, phi) - c.torsMin))) return phi;
+  }
+  return M_PI;
+}
+
+__device__ __forceinline__ void unit3(double v[3])
+{
+  const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= n;
+  v[1] /= n;
+  v[2] /= n;
+}
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// trial position of bead i given beads 0 .. i-1 (mol, 3 per bead): bond, sampled bend, azimuth or torsion
+__device__ void chainTrial(const McRun& R, const Comp& c, int sys, unsigned long long step, int i, int j,
+                           const double* mol, double out[3])
+{
+  if (i == 1)
+  {
+    double u, v;
+    uniChain(R, sys, step, i, j, 0, 0, u, v);
+    const double z = 2.0 * u - 1.0, r = sqrt(fmax(0.0, 1.0 - z * z)), ps = 2.0 * M_PI * v;
+    out[0] = mol[0] + c.bond * r * cos(ps);
+    out[1] = mol[1] + c.bond * r * sin(ps);
+    out[2] = mol[2] + c.bond * z;
+    return;
+  }
+  const double* B = mol + 3 * (i - 2);
+  const double* Cp = mol + 3 * (i - 1);
+  double e[3] = {Cp[0] - B[0], Cp[1] - B[1], Cp[2] - B[2]};
+  unit3(e);
+  double m[3], n[3];
+  if (i == 2)
+  {
+    const double ax[3] = {fabs(e[0]) < 0.9 ? 1.0 : 0.0, fabs(e[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+    cross3(e, ax, m);
+    unit3(m);
+    cross3(e, m, n);
+  }
+  else
+  {
+    const double* A = mol + 3 * (i - 3);
+    const double b1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    cross3(b1, e, n);
+    unit3(n);
+    cross3(n, e, m);
+  }
+  const double th = sampleBend(R, c, sys, step, i, j);
+  double ph;
+  if (i == 2)
+  {
+    double u, v;
+    uniChain(R, sys, step, i, j, 0, 0, u, v);
+    ph = 2.0 * M_PI * u;
+  }
+  else
+    ph = sampleTorsion(R, c, sys, step, i, j);
+  const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
+  for (int d = 0; d < 3; ++d) out[d] = Cp[d] + c.bond * (-ct * e[d] + st * cp * m[d] + st * sp * n[d]);
+}
+
+// bend and torsion energy of bead i against beads i-3 .. i-1 (mol, 3 per bead)
+__device__ double chainBonded(const Comp& c, const double* mol, int i)
+{
+  double e = 0.0;
+  if (i >= 2)
+  {
+    const double* B = mol + 3 * (i - 2);
+    const double* Cp = mol + 3 * (i - 1);
+    const double* D = mol + 3 * i;
+    double u[3] = {B[0] - Cp[0], B[1] - Cp[1], B[2] - Cp[2]}, v[3] = {D[0] - Cp[0], D[1] - Cp[1], D[2] - Cp[2]};
+    unit3(u);
+    unit3(v);
+    const double th = acos(fmax(-1.0, fmin(1.0, u[0] * v[0] + u[1] * v[1] + u[2] * v[2])));
+    const double d = th - c.theta0;
+    e += 0.5 * c.kbend * d * d;
+  }
+  if (i >= 3)
+  {
+    const double* A = mol + 3 * (i - 3);
+    const double* B = mol + 3 * (i - 2);
+    const double* Cp = mol + 3 * (i - 1);
+    const double* D = mol + 3 * i;
+    const double b1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, b2[3] = {Cp[0] - B[0], Cp[1] - B[1], Cp[2] - B[2]},
+                 b3[3] = {D[0] - Cp[0], D[1] - Cp[1], D[2] - Cp[2]};
+    double n1[3], n2[3], m1[3], e2[3] = {b2[0], b2[1], b2[2]};
+    cross3(b1, b2, n1);
+    cross3(b2, b3, n2);
+    unit3(e2);
+    cross3(n1, e2, m1);
+    const double x = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+    const double y = m1[0] * n2[0] + m1[1] * n2[1] + m1[2] * n2[2];
+    e += chainTorsionEnergy(c, atan2(y, x));
+  }
+  return e;
+}
+
+// intra-molecular LJ of a site of bead i against beads 0 .. i-4
+__device__ double chainIntra(const SysDev& S, const Comp& c, const double* mol, int i, const double* p, bool& ov)
+{
+  double e = 0.0;
+  for (int b = 0; b + 3 < i; ++b)
+  {
+    const double dx = p[0] - mol[3 * b], dy = p[1] - mol[3 * b + 1], dz = p[2] - mol[3 * b + 2];
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < S.rcv2Mol)
+    {
+      const double v = rgd::vdw0(S.pairs[c.type[b] * S.ntypes + c.type[i]], 1.0, rr);
+      ov = ov || v > S.overlap;
+      e += v;
+    }
+  }
+  return e;
+}
+
+// Chain growth (retrace >= 0: the existing molecule is trial 0 of every step): the first bead as for a rigid
+// molecule, then kOrient trials for each of beads 1 .. na-1.  Writes the chain into W.mol; Uext is the external
+// plus intra-molecular (non-bonded and bonded) energy.
+__device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
+                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+{
+  const int K = R.kOrient;
+  const int own = retrace >= 0 ? retrace : -2;
+  double fx, fy, fz, Ufirst, lw, uSel2;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, fx, fy, fz, Ufirst, lw, uSel2)) return false;
+  if (lane == 0)
+  {
+    W.mol[0] = fx;
+    W.mol[1] = fy;
+    W.mol[2] = fz;
+  }
+  waveSyncLds();
+  logW = lw;
+  Uext = Ufirst;
+  for (int i = 1; i < c.na; ++i)
+  {
+    double p[3] = {0.0, 0.0, 0.0};
+    if (lane < K)
+    {
+      if (retrace >= 0 && lane == 0)
+      {
+        const int a0 = retrace * MC_MAXATOMS + i;
+        p[0] = S.ax[a0];
+        p[1] = S.ay[a0];
+        p[2] = S.az[a0];
+      }
+      else
+        chainTrial(R, c, sys, step, i, lane, W.mol, p);
+      W.pos[3 * lane] = p[0];
+      W.pos[3 * lane + 1] = p[1];
+      W.pos[3 * lane + 2] = p[2];
+    }
+    waveSyncLds();
+    double Ui;
+    bool ovi;
+    const int bi[1] = {i};
+    trialSetEnergies(S, nads, c, bi, 1, K, W.pos, own, -2, lane, Ui, ovi);
+    if (lane < K) Ui += chainIntra(S, c, W.mol, i, p, ovi);
+    double us, unused;
+    uni(R, sys, step, SLOT_CHAIN - 16 + i, us, unused);
+    int sel;
+    waveRosenbluth(Ui, ovi, 0.0, K, R.beta, us, lane, sel, lw);
+    if (retrace >= 0) sel = 0;
+    if (sel < 0) return false;
+    if (retrace >= 0 && __shfl(ovi ? 1 : 0, 0, 64)) return false;
+    const double sx = __shfl(p[0], sel, 64), sy = __shfl(p[1], sel, 64), sz = __shfl(p[2], sel, 64);
+    const double Usel = __shfl(Ui, sel, 64);
+    waveSyncLds();  // every lane has read W.pos / W.mol of this step
+    if (lane == 0)
+    {
+      W.mol[3 * i] = sx;
+      W.mol[3 * i + 1] = sy;
+      W.mol[3 * i + 2] = sz;
+    }
+    waveSyncLds();
+    logW += lw;
+    if (logW < R.logMinW) return false;
+    Uext += Usel + chainBonded(c, W.mol, i);
+  }
+  return true;
+}
+
+__device__ __forceinline__ bool growMolecule(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys,
+                                             int nads, unsigned long long step, int retrace, int lane, WaveScratch& W,
+                                             double& logW, double& Uext)
+{
+  return c.flexible ? growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext)
+                    : growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
+}
+
+
 // Ewald energy change of inserting (+1) or removing (-1) the atoms in W.mol of component c, incl. the constant
 // self and intra-molecular exclusion terms of the rigid body.
 __device__ double ewaldInsertRemove(const McRun& R, const SysDev& S, double netCharge, const Comp& c, int sign, int lane,
@@ -524,8 +788,14 @@ __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, con
   const McSys& Y = *Yp;
   const Comp& c = R.comp[comp];
   double logW = -INFINITY, U = 0.0, de = 0.0;
-  if (growRigid(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
+  if (growMolecule(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
     de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], false);
+  else
+  {
+    // a failed growth (every trial of a step overlapping) has weight zero, whatever partial sums it left
+    logW = -INFINITY;
+    U = 0.0;
+  }
   if (lane == 0)
   {
     outLogW[i] = logW;
@@ -725,49 +995,20 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     return;
   }
   const double fugVol = R.beta * c.fugacity * S.box.volume;
-  if (uMove < pRot + 0.5 * (1.0 - pRot))
-  {
-    // CBMC insertion (src/raspakit/mc_moves/component/insertion_cbmc.cpp:34-200)
-    L.trials[0] += 1;
-    if (L.nmol >= R.maxMolecules) return;
-    double logW, Uext;
-    if (!growRigid(R, S, Y, c, sys, L.nads, step, -1, lane, W, logW, Uext)) return;
-    const double de = ewaldInsertRemove(R, S, L.netCharge, c, +1, lane, W, true);
-    const int nc = L.ncomp[ci];
-    // Pacc = exp(-beta dE_Ewald) beta f V / (N + 1) W / W_IG, W_IG = 1 (insertion_cbmc.cpp:157-168)
-    const double logPacc = -R.beta * de + log(fugVol / double(nc + 1)) + logW;
-    if (log(uAcc) < logPacc)
-    {
-      const int m = L.nmol;
-      if (lane < MC_MAXATOMS)
-      {
-        const int a0 = m * MC_MAXATOMS + lane;
-        const bool real = lane < c.na;
-        S.ax[a0] = real ? W.mol[3 * lane] : 0.0;
-        S.ay[a0] = real ? W.mol[3 * lane + 1] : 0.0;
-        S.az[a0] = real ? W.mol[3 * lane + 2] : 0.0;
-        S.aq[a0] = real ? c.q[lane] : 0.0;
-        S.at[a0] = real ? c.type[lane] : -1;
-        S.amol[a0] = m;
-      }
-      if (lane == 0) Y.comp[m] = ci;
-      if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
-      L.nmol = m + 1;
-      L.ncomp[ci] = nc + 1;
-      L.nads = (m + 1) * MC_MAXATOMS;
-      L.netCharge += c.qsum;
-      L.U += Uext + de;
-      L.accepted[0] += 1;
-    }
-    return;
-  }
-  // CBMC deletion (src/raspakit/mc_moves/component/deletion_cbmc.cpp:67-174)
-  L.trials[1] += 1;
+  // CBMC insertion (src/raspakit/mc_moves/component/insertion_cbmc.cpp:34-200) or deletion
+  // (src/raspakit/mc_moves/component/deletion_cbmc.cpp:67-174); one growth call serves both (retrace = m)
+  const bool insert = uMove < pRot + 0.5 * (1.0 - pRot);
+  L.trials[insert ? 0 : 1] += 1;
   const int nc = L.ncomp[ci];
-  if (nc == 0) return;
-  const int kth = min(int(uMol * nc), nc - 1);
   int m = -1;
+  if (insert)
   {
+    if (L.nmol >= R.maxMolecules) return;
+  }
+  else
+  {
+    if (nc == 0) return;
+    const int kth = min(int(uMol * nc), nc - 1);
     int seen = 0;
     for (int base = 0; base < L.nmol && m < 0; base += 64)
     {
@@ -783,13 +1024,42 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
       }
       seen += cnt;
     }
+    if (m < 0) return;
   }
-  if (m < 0) return;
   double logW, Uext;
-  if (!growRigid(R, S, Y, c, sys, L.nads, step, m, lane, W, logW, Uext)) return;
-  const double de = ewaldInsertRemove(R, S, L.netCharge, c, -1, lane, W, true);
+  if (!growMolecule(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, logW, Uext)) return;
+  const double de = ewaldInsertRemove(R, S, L.netCharge, c, insert ? +1 : -1, lane, W, true);
+  if (insert)
+  {
+    // Pacc = exp(-beta dE_Ewald) beta f V / (N + 1) W / W_IG (insertion_cbmc.cpp:157-168); W_IG = 1 for rigid bodies
+    const double logPacc = -R.beta * de + log(fugVol / double(nc + 1)) + logW - c.logWIG;
+    if (log(uAcc) < logPacc)
+    {
+      const int mn = L.nmol;
+      if (lane < MC_MAXATOMS)
+      {
+        const int a0 = mn * MC_MAXATOMS + lane;
+        const bool real = lane < c.na;
+        S.ax[a0] = real ? W.mol[3 * lane] : 0.0;
+        S.ay[a0] = real ? W.mol[3 * lane + 1] : 0.0;
+        S.az[a0] = real ? W.mol[3 * lane + 2] : 0.0;
+        S.aq[a0] = real ? c.q[lane] : 0.0;
+        S.at[a0] = real ? c.type[lane] : -1;
+        S.amol[a0] = mn;
+      }
+      if (lane == 0) Y.comp[mn] = ci;
+      if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
+      L.nmol = mn + 1;
+      L.ncomp[ci] = nc + 1;
+      L.nads = (mn + 1) * MC_MAXATOMS;
+      L.netCharge += c.qsum;
+      L.U += Uext + de;
+      L.accepted[0] += 1;
+    }
+    return;
+  }
   // Pacc = exp(-beta dE_Ewald) N / (beta f V) W_IG / W (deletion_cbmc.cpp:159-161)
-  const double logPacc = -R.beta * de + log(double(nc) / fugVol) - logW;
+  const double logPacc = -R.beta * de + log(double(nc) / fugVol) - logW + c.logWIG;
   if (log(uAcc) < logPacc)
   {
     if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
@@ -923,6 +1193,22 @@ __global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ 
         if (S.useCharge && rr < S.rcc2) ei += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.aq[b] * c.q[a]);
       }
       e += ef + rgd::waveSum(ei);
+    }
+    if (c.flexible)
+    {
+      // intra-molecular energy of the chain: bend, torsion and LJ between beads more than three bonds apart
+      double mol[3 * MC_MAXATOMS];
+      for (int a = 0; a < c.na; ++a)
+      {
+        mol[3 * a] = S.ax[m * MC_MAXATOMS + a];
+        mol[3 * a + 1] = S.ay[m * MC_MAXATOMS + a];
+        mol[3 * a + 2] = S.az[m * MC_MAXATOMS + a];
+      }
+      for (int i = 1; i < c.na; ++i)
+      {
+        bool o = false;
+        e += chainIntra(S, c, mol, i, mol + 3 * i, o) + chainBonded(c, mol, i);
+      }
     }
   }
   // Ewald Fourier energy of the adsorbates from the stored structure factor

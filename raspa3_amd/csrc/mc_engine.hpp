@@ -3,7 +3,9 @@
 
 #include "mc_internal.hpp"
 
-#define MC_MAXATOMS 4   // atoms per rigid molecule (slots per molecule in the adsorbate arrays)
+#define MC_MAXATOMS 8   // atom slots per molecule in the adsorbate arrays (flexible chains up to 8 beads)
+#define MC_MAXRIGID 4   // atoms of a rigid molecule
+#define MC_MAXREJ 128   // rejection attempts per sampled bend or torsion angle of a chain
 #define MC_MAXCOMP 2    // components per system
 #define MC_BLOCKS 5     // block averages (src/raspakit/properties/property_block_average.ixx)
 #define MC_MAXTRIALS 64 // first-bead / orientation trials (one lane each)
@@ -26,6 +28,12 @@ struct Comp
   double eIntra;      // C alpha/sqrt(pi) sum q^2 + sum_pairs C qi qj erf(alpha r)/r  (internal units)
   double molFraction;
   double fugacity;    // y phi P in internal units
+  // flexible linear chain (first bead 0, grown 1, 2, ...): bond length, harmonic bend (theta0 in rad, k in internal
+  // units / rad^2), torsion series c0 + c1 (1 + cos p) + c2 (1 - cos 2p) + c3 (1 + cos 3p) and its minimum, log of
+  // the ideal-gas Rosenbluth weight of the growth
+  int flexible;
+  int pad2;
+  double bond, theta0, kbend, tors[4], torsMin, logWIG;
 };
 
 // Tabulated first-bead proposal: probabilities p_v over nvox voxels of the cell, alias table, and
@@ -89,9 +97,9 @@ struct McRun
 // LDS of one wavefront in the Monte Carlo kernels: trial atom positions, Ewald eik tables and the grown molecule.
 struct WaveScratch
 {
-  double pos[64 * 3 * (MC_MAXATOMS - 1) > 64 * 3 ? 64 * 3 * (MC_MAXATOMS - 1) : 64 * 3];
-  double at[4 * 2 * MC_MAXATOMS];
-  double2 tab[2 * MC_MAXATOMS * 3 * (EW_MAXK + 1)];
+  double pos[64 * 3 * (MC_MAXRIGID - 1)];
+  double at[4 * 2 * MC_MAXRIGID];
+  double2 tab[2 * MC_MAXRIGID * 3 * (EW_MAXK + 1)];
   double mol[3 * MC_MAXATOMS];
 };
 

@@ -70,8 +70,9 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
   {
     const rg_component_desc& d = comps[c];
     Comp& C = R.comp[c];
-    if (d.n_atoms < 1 || d.n_atoms > MC_MAXATOMS || d.first_bead < 0 || d.first_bead >= d.n_atoms)
-      return fail(ctx, RG_ERR_UNSUPPORTED, "component %d: 1..%d atoms", c, MC_MAXATOMS);
+    const int maxAtoms = d.flexible ? MC_MAXATOMS : MC_MAXRIGID;
+    if (d.n_atoms < 1 || d.n_atoms > maxAtoms || d.first_bead < 0 || d.first_bead >= d.n_atoms)
+      return fail(ctx, RG_ERR_UNSUPPORTED, "component %d: 1..%d atoms", c, maxAtoms);
     C.na = d.n_atoms;
     C.first = d.first_bead;
     C.qsum = 0.0;
@@ -100,6 +101,27 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
       }
     }
     C.eIntra = ff->use_charge ? self + excl : 0.0;
+    C.flexible = d.flexible ? 1 : 0;
+    if (C.flexible)
+    {
+      if (d.first_bead != 0 || C.charged || !(d.bond_length > 0.0) || d.bend_k_K < 0.0)
+        return fail(ctx, RG_ERR_UNSUPPORTED,
+                    "component %d: flexible chains grow from bead 0, are uncharged and need a positive bond length", c);
+      C.bond = d.bond_length;
+      C.theta0 = d.bend_theta0_deg * M_PI / 180.0;
+      C.kbend = d.bend_k_K * u.kelvinToEnergy;
+      for (int k = 0; k < 4; ++k) C.tors[k] = d.torsion_K[k] * u.kelvinToEnergy;
+      C.logWIG = d.log_ideal_rosenbluth;
+      // minimum of the torsion series on a fine grid: the rejection sampler accepts with exp(-beta (U - U_min))
+      C.torsMin = INFINITY;
+      for (int k = 0; k <= 7200; ++k)
+      {
+        const double ph = -M_PI + 2.0 * M_PI * k / 7200.0;
+        const double e = C.tors[0] + C.tors[1] * (1.0 + std::cos(ph)) + C.tors[2] * (1.0 - std::cos(2.0 * ph)) +
+                         C.tors[3] * (1.0 + std::cos(3.0 * ph));
+        C.torsMin = std::min(C.torsMin, e);
+      }
+    }
     C.molFraction = d.mol_fraction / ysum;
     C.fugacity = C.molFraction * p->pressure_pa / pressureConversion;  // fugacity coefficient 1
   }

@@ -130,7 +130,7 @@ def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx):
     comp = w.co2_reference(spec)
     params = w.mc_params(pressure_pa=1e5, seed=5, max_molecules=128, production_cycles=15)
     systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
-                         capacity=4 * 128) for _ in range(2)]
+                         capacity=8 * 128) for _ in range(2)]
     mc = ra.Mc(gpu_ctx, systems, [comp], params)
     mc.run(5, sample=False, graph=False)
     mc.run(15, sample=True, graph=True)
@@ -154,7 +154,7 @@ def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
     comps = [w.co2_reference(spec, 0.8), w.n2_component(spec, 0.2)]
     params = w.mc_params(pressure_pa=5e5, seed=77, max_molecules=128, production_cycles=10)
     s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
-                  capacity=4 * 128)
+                  capacity=8 * 128)
     mc = ra.Mc(gpu_ctx, [s], comps, params)
     rng = np.random.default_rng(3)
     n3 = [8, 8, 8]
@@ -181,7 +181,7 @@ def test_graph_replay_equals_eager(gpu_ctx):
     runs = []
     for graph in (False, True):
         systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul,
-                             kmax=(6, 6, 6), capacity=4 * 64) for _ in range(3)]
+                             kmax=(6, 6, 6), capacity=8 * 64) for _ in range(3)]
         mc = ra.Mc(gpu_ctx, systems, [comp], params)
         trace = []
         for _ in range(6):
@@ -211,3 +211,55 @@ def test_molecule_weights_match_oracle(gpu_ctx):
         olw = om.molecule_weights(0, frac, m, 17)
         assert close(lw, olw, rel=1e-9, abs_=1e-9), m
     assert np.isfinite(lw).any() and np.isneginf(lw).any()
+
+
+def hexane_itq29(ctx, spacing=0.3):
+    spec = w.forcefield_hexane_zeolite()
+    spec.use_charge = False
+    fw, unit = w.framework_itq29(spec, 3)
+    ff = ra.ForceField(ctx, spec)
+    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw.pos[:72], fw.type[:72], fw.charge[:72]))
+    dims = ra.grid_dims(unit, spacing)
+    types = [spec.index("CH3"), spec.index("CH2")]
+    grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
+    ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()])
+    host = {t: g.download().reshape(dims[2], dims[1], dims[0], 8) for t, g in grids.items()}
+    return spec, fw, unit, ff, ra.Framework(ctx, fw), dims, grids, host
+
+
+def test_widom_hexane_replays_oracle(gpu_ctx):
+    """Flexible-chain growth (n-hexane, 6 beads, bend and torsion sampled by rejection) in ITQ-29: the GPU's Widom
+    insertions equal the oracle's, trial for trial."""
+    print("case: widom hexane", flush=True)
+    spec, fw, unit, ff, f, dims, grids, host = hexane_itq29(gpu_ctx)
+    comp = w.hexane_component(spec, 0.3)
+    s = ra.System(gpu_ctx, ff, fw.cell, True, framework=f, grids=grids, kmax=(0, 0, 0), capacity=64)
+    params = w.mc_params(seed=31)
+    lw, u, _ = ra.widom(gpu_ctx, s, comp, params, 5, 256)
+    om = oracle.MC(spec, fw, [comp], params, kmax=(0, 0, 0), grids=(host, None, dims, unit))
+    olw, ou, _ = om.widom(0, 5, 256)
+    assert close(lw, olw, rel=1e-9, abs_=1e-9) and close(u, ou, rel=1e-9, abs_=1e-6)
+    assert np.isfinite(lw).sum() > 100
+
+
+def test_gcmc_hexane_replays_oracle_trajectory(gpu_ctx):
+    """Two n-hexane GCMC systems in ITQ-29 (chain insertion, deletion with retrace, translation, rotation) for
+    4 + 12 cycles replay the oracle move by move; the running energy (with the chains' bend, torsion and
+    intra-molecular LJ) equals the recomputed total."""
+    print("case: gcmc hexane replay", flush=True)
+    spec, fw, unit, ff, f, dims, grids, host = hexane_itq29(gpu_ctx)
+    comp = w.hexane_component(spec, 0.3)
+    params = w.mc_params(pressure_pa=10.0, seed=13, max_molecules=64, production_cycles=12)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, True, framework=f, grids=grids, kmax=(0, 0, 0), capacity=8 * 64)
+               for _ in range(2)]
+    mc = ra.Mc(gpu_ctx, systems, [comp], params)
+    mc.run(4, sample=False, graph=False)
+    mc.run(12, sample=True, graph=True)
+    for sidx in range(2):
+        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (0, 0, 0), host, None, dims, unit, (4, 12))
+        assert g["accepted"][0] > 0 and g["n_molecules"] > 0
+    tot = mc.total_energy()
+    for sidx in range(2):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
