@@ -24,6 +24,8 @@ Second half of the metric, in the same JSON line:
   their standard errors (over systems), and whether they agree within 2 sigma.
 * mlp: training throughput of the proposal network (BF16 MFMA) on 2^21 points of the configuration-1 CO2 grids at
   300 K, samples/s and MFMA TFLOP/s at 1.08 MFLOP per training sample.
+* gcmc_uneven: batched CBMC over 64 systems of uneven size and loading (ITQ-29 supercells of 1944-7200 sites),
+  flexible n-hexane chains and rigid CO2 with the Ewald difference at kmax 8, one launch per cycle.
 """
 import argparse
 import json
@@ -346,6 +348,65 @@ def bench_gcmc(ctx, args):
                 unit="accepted insertions/s")
 
 
+def bench_gcmc_uneven(ctx, args):
+    """Batched CBMC over uneven systems: n-hexane (flexible chains) + CO2 (Ewald kmax 8) at 300 K and 1000 Pa
+    (mole fractions 0.2 / 0.8) in ITQ-29 supercells of six sizes (1944-7200 sites), one launch per cycle."""
+    import raspa3_amd as ra
+    from raspa3_amd import workloads as w
+    from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
+
+    spec = w.forcefield_hexane_zeolite()
+    ff = ra.ForceField(ctx, spec)
+    fw3, unit = w.framework_itq29(spec, 3)
+    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw3.pos[:72], fw3.type[:72], fw3.charge[:72]))
+    dims = ra.grid_dims(unit, 0.2)
+    types = [spec.index(n) for n in ("CH3", "CH2", "C_co2", "O_co2")]
+    grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
+    coul = ra.Grid(ctx, unit, True, dims)
+    ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
+                   [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    # ideal-gas Rosenbluth weight of the hexane growth: Widom insertions into an empty box
+    empty = ra.System(ctx, ff, np.diag([40.0] * 3), True, kmax=(0, 0, 0), capacity=64)
+    lw0, _, _ = ra.widom(ctx, empty, w.hexane_component(spec), w.mc_params(seed=3), 0, 1 << 16)
+    log_wig = float(np.log(np.exp(lw0).mean()))
+    comps = [w.hexane_component(spec, log_wig, 0.2), w.co2_reference(spec, 0.8)]
+    shapes = [(3, 3, 3), (3, 3, 4), (3, 4, 4), (4, 4, 4), (4, 4, 5), (4, 5, 5)]
+    frameworks = {}
+    for sh in shapes:
+        fwk, _ = w.framework_itq29(spec, sh)
+        frameworks[sh] = (fwk, ra.Framework(ctx, fwk))
+    nsys = args.uneven_systems
+    systems = []
+    for k in range(nsys):
+        fwk, fo = frameworks[shapes[k % len(shapes)]]
+        systems.append(ra.System(ctx, ff, fwk.cell, True, framework=fo, grids=grids, coulomb_grid=coul,
+                                 kmax=(8, 8, 8), capacity=8 * 256))
+    params = w.mc_params(pressure_pa=1000.0, seed=77, max_molecules=256,
+                         production_cycles=args.uneven_production)
+    mc = ra.Mc(ctx, systems, comps, params)
+    mc.run(args.uneven_equilibration, sample=False, graph=True)
+    s0 = [mc.state(k) for k in range(nsys)]
+    t = time.perf_counter()
+    mc.run(args.uneven_production, sample=True, graph=True)
+    dt = time.perf_counter() - t
+    s1 = [mc.state(k) for k in range(nsys)]
+    acc = sum(s1[k]["accepted"][0] - s0[k]["accepted"][0] for k in range(nsys))
+    steps = sum(s1[k]["steps"] - s0[k]["steps"] for k in range(nsys))
+    n = np.array([s["n_per_component"] for s in s1])
+    tot = mc.total_energy()
+    drift = max(abs(tot[k] - s1[k]["energy"]) for k in range(nsys))
+    mc.close()
+    log(f"gcmc uneven: {acc / dt:.4e} accepted insertions/s, {steps / dt:.4e} MC steps/s, hexane loadings "
+        f"{n[:, 0].min()}-{n[:, 0].max()}")
+    return dict(workload=f"{nsys} systems, n-hexane (flexible, 0.2) + CO2 (rigid, Ewald kmax 8, 0.8) at 300 K and "
+                         f"1000 Pa in ITQ-29 supercells {shapes} (1944-7200 sites), {args.uneven_equilibration} + "
+                         f"{args.uneven_production} cycles",
+                accepted_insertions_per_s=acc / dt, mc_steps_per_s=steps / dt, seconds=round(dt, 4),
+                loadings_hexane=[int(n[:, 0].min()), int(n[:, 0].max())],
+                loadings_co2=[int(n[:, 1].min()), int(n[:, 1].max())], log_ideal_rosenbluth_hexane=log_wig,
+                max_energy_drift=float(drift), unit="accepted insertions/s")
+
+
 def bench_mlp(ctx, g1, args):
     """Proposal-network training throughput on configuration-1 CO2 grid data (2^21 points, 300 K)."""
     import raspa3_amd as ra
@@ -424,6 +485,9 @@ def main():
     ap.add_argument("--voxels", type=int, default=64)
     ap.add_argument("--mlp-epochs", type=int, default=4)
     ap.add_argument("--mlp-batch", type=int, default=4096)
+    ap.add_argument("--uneven-systems", type=int, default=64)
+    ap.add_argument("--uneven-equilibration", type=int, default=300)
+    ap.add_argument("--uneven-production", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build path shards over "
@@ -443,6 +507,7 @@ def main():
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)
         extra["gcmc"] = bench_gcmc(ctx, args)
+        extra["gcmc_uneven"] = bench_gcmc_uneven(ctx, args)
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(g1["fw"], g1["spec"], g1["dims"], args.cpu_seconds)

@@ -239,7 +239,7 @@ typedef struct rg_mc_params
   double max_translation;   /* A */
   double max_rotation;      /* rad */
   int min_steps_per_cycle;  /* steps per cycle = max(this, N) (src/raspakit/drivers/monte_carlo.cpp:301) */
-  int max_molecules;
+  int max_molecules;        /* per system; each system's capacity_atoms must be >= max_molecules * max(n_atoms) */
   unsigned long long seed;  /* Philox4x32-10 key; trials are addressed by (system, step, slot) */
   long long production_cycles; /* cycles over which the 5 block averages are spread */
 } rg_mc_params;
@@ -275,7 +275,8 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
  * On return the systems' adsorbate arrays, structure factors, adsorbate count and net charge are the final state. */
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);
 rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
-/* Adsorbate atoms of a system (8 slots per molecule; unused slots have type -1) and the component of each molecule. */
+/* Adsorbate atoms of a system (max(n_atoms) of the components slots per molecule; unused slots have type -1)
+ * and the component of each molecule. */
 rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacity, double* pos, int* type,
                            int* component, int* n_out);
 /* Training data of the learned proposal: the orientation-averaged Boltzmann factor of a rigid molecule with its

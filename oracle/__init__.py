@@ -307,7 +307,7 @@ def _mc_lib():
     return L
 
 
-SLOTS = 8          # atom slots per molecule in the Monte Carlo adsorbate arrays
+MAX_ATOMS = 8      # most atoms per Monte Carlo component; a system's adsorbate arrays hold max(n_atoms) per molecule
 COMP_INT, COMP_DBL = 11, 41  # component record sizes of orc_mc_create
 
 
@@ -343,6 +343,7 @@ class MC:
                 self._keep.append(cg)
             gdims = _n(dims)
             self._keep.append(gdims)
+        self.slots = max(int(d["n_atoms"]) for d in comps)
         ci = np.zeros(COMP_INT * len(comps), np.int32)
         cd = np.zeros(COMP_DBL * len(comps))
         for c, d in enumerate(comps):
@@ -396,7 +397,7 @@ class MC:
         t = np.zeros(capacity, np.int32)
         c = np.zeros(capacity, np.int32)
         n = _mc_lib().orc_mc_adsorbates(self.h, _d(pos), _i(t), _i(c))
-        return pos[:n], t[:n], c[:n // SLOTS]
+        return pos[:n], t[:n], c[:n // self.slots]
 
     def grow(self, component, step, n_atoms):
         """One growth of a molecule at move 'step' (no acceptance): positions, log W, energy (None if it failed)."""

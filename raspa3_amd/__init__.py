@@ -357,6 +357,7 @@ class Mc(_Handle):
         self.systems = systems
         arr = (C.c_void_p * len(systems))(*[s.h.value for s in systems])
         descs = (L.ComponentDesc * len(comps))(*[component_desc(c) for c in comps])
+        self.slots = max(int(c["n_atoms"]) for c in comps)  # atom slots per molecule in the adsorbate arrays
         p = mc_params(params)
         h = C.c_void_p()
         L.check(lib().rg_mc_create(ctx.h, len(systems), arr, len(comps), descs, C.byref(p), C.byref(h)), ctx.h,
@@ -389,7 +390,7 @@ class Mc(_Handle):
         n = C.c_int()
         L.check(lib().rg_mc_adsorbates(self.ctx.h, self.h, system, capacity, L.d(pos), L.i(t), L.i(c), C.byref(n)),
                 self.ctx.h, "rg_mc_adsorbates")
-        return pos[:n.value], t[:n.value], c[:n.value // L.SLOTS]
+        return pos[:n.value], t[:n.value], c[:n.value // self.slots]
 
     def total_energy(self):
         out = np.zeros(len(self.systems))

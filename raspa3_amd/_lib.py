@@ -101,7 +101,7 @@ def check(status, ctx=None, what=""):
         raise RGError(f"{what}: status {status}: {msg}")
 
 
-SLOTS = 8  # atom slots per molecule in the Monte Carlo adsorbate arrays
+MAX_ATOMS = 8  # most atoms per Monte Carlo component; adsorbate arrays hold max(n_atoms) slots per molecule
 
 
 class ComponentDesc(C.Structure):

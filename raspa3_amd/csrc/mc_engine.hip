@@ -389,7 +389,7 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
   {
     if (retrace >= 0 && lane == 0)
     {
-      const int a0 = retrace * MC_MAXATOMS + fb;
+      const int a0 = retrace * R.slots + fb;
       p[0] = S.ax[a0];
       p[1] = S.ay[a0];
       p[2] = S.az[a0];
@@ -456,7 +456,7 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
     {
       for (int s = 0; s < n; ++s)
       {
-        const int a0 = retrace * MC_MAXATOMS + idx[s];
+        const int a0 = retrace * R.slots + idx[s];
         W.pos[3 * s] = S.ax[a0];
         W.pos[3 * s + 1] = S.ay[a0];
         W.pos[3 * s + 2] = S.az[a0];
@@ -700,7 +700,7 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
     {
       if (retrace >= 0 && lane == 0)
       {
-        const int a0 = retrace * MC_MAXATOMS + i;
+        const int a0 = retrace * R.slots + i;
         p[0] = S.ax[a0];
         p[1] = S.ay[a0];
         p[2] = S.az[a0];
@@ -741,12 +741,14 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
   return true;
 }
 
+// CHAIN = false compiles the chain growth out of kernels whose components are all rigid (fewer registers).
+template <bool CHAIN>
 __device__ __forceinline__ bool growMolecule(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys,
                                              int nads, unsigned long long step, int retrace, int lane, WaveScratch& W,
                                              double& logW, double& Uext)
 {
-  return c.flexible ? growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext)
-                    : growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
+  if (CHAIN && c.flexible) return growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
+  return growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
 }
 
 
@@ -775,6 +777,7 @@ __device__ double ewaldInsertRemove(const McRun& R, const SysDev& S, double netC
 // ---------------------------------------------------------------------------------------------------------------
 // Widom insertions: one wavefront per insertion i (the "step" of the stream), system state unchanged.
 // ---------------------------------------------------------------------------------------------------------------
+template <bool CHAIN>
 __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, const SysDev* __restrict__ Sp,
                                                const McSys* __restrict__ Yp, int comp, long long first, int count,
                                                double* outLogW, double* outU, double* outEwald)
@@ -788,7 +791,7 @@ __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, con
   const McSys& Y = *Yp;
   const Comp& c = R.comp[comp];
   double logW = -INFINITY, U = 0.0, de = 0.0;
-  if (growMolecule(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
+  if (growMolecule<CHAIN>(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
     de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], false);
   else
   {
@@ -866,6 +869,7 @@ __global__ __launch_bounds__(256) void k_molecule_weights(const SysDev* __restri
   if (active && o == 0) logW[i] = (m == -INFINITY) ? -INFINITY : m + log(sum / M);
 }
 
+template <bool CHAIN>
 __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W)
 {
   const unsigned long long step = (unsigned long long)L.step;
@@ -901,7 +905,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     // old positions into W.pos[0 .. na), new ones into W.pos[na .. 2na)
     if (lane < cm.na)
     {
-      const int a0 = m * MC_MAXATOMS + lane;
+      const int a0 = m * R.slots + lane;
       const double ox = S.ax[a0], oy = S.ay[a0], oz = S.az[a0];
       W.pos[3 * lane] = ox;
       W.pos[3 * lane + 1] = oy;
@@ -916,7 +920,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
       else
       {
         // rotation by theta about a random unit axis through the first bead (Rodrigues' formula)
-        const int f0 = m * MC_MAXATOMS + cm.first;
+        const int f0 = m * R.slots + cm.first;
         const double cx = S.ax[f0], cy = S.ay[f0], cz = S.az[f0];
         const double zc = 2.0 * d0 - 1.0, ph = 2.0 * M_PI * d1, sr = sqrt(fmax(0.0, 1.0 - zc * zc));
         const double kx = sr * cos(ph), ky = sr * sin(ph), kz = zc;
@@ -983,7 +987,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     {
       if (lane < cm.na)
       {
-        const int a0 = m * MC_MAXATOMS + lane;
+        const int a0 = m * R.slots + lane;
         S.ax[a0] = W.pos[3 * (cm.na + lane)];
         S.ay[a0] = W.pos[3 * (cm.na + lane) + 1];
         S.az[a0] = W.pos[3 * (cm.na + lane) + 2];
@@ -1027,7 +1031,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     if (m < 0) return;
   }
   double logW, Uext;
-  if (!growMolecule(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, logW, Uext)) return;
+  if (!growMolecule<CHAIN>(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, logW, Uext)) return;
   const double de = ewaldInsertRemove(R, S, L.netCharge, c, insert ? +1 : -1, lane, W, true);
   if (insert)
   {
@@ -1036,9 +1040,9 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     if (log(uAcc) < logPacc)
     {
       const int mn = L.nmol;
-      if (lane < MC_MAXATOMS)
+      if (lane < R.slots)
       {
-        const int a0 = mn * MC_MAXATOMS + lane;
+        const int a0 = mn * R.slots + lane;
         const bool real = lane < c.na;
         S.ax[a0] = real ? W.mol[3 * lane] : 0.0;
         S.ay[a0] = real ? W.mol[3 * lane + 1] : 0.0;
@@ -1051,7 +1055,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
       if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
       L.nmol = mn + 1;
       L.ncomp[ci] = nc + 1;
-      L.nads = (mn + 1) * MC_MAXATOMS;
+      L.nads = (mn + 1) * R.slots;
       L.netCharge += c.qsum;
       L.U += Uext + de;
       L.accepted[0] += 1;
@@ -1067,9 +1071,9 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     const int last = L.nmol - 1;
     if (m != last)
     {
-      if (lane < MC_MAXATOMS)
+      if (lane < R.slots)
       {
-        const int d = m * MC_MAXATOMS + lane, s = last * MC_MAXATOMS + lane;
+        const int d = m * R.slots + lane, s = last * R.slots + lane;
         S.ax[d] = S.ax[s];
         S.ay[d] = S.ay[s];
         S.az[d] = S.az[s];
@@ -1080,7 +1084,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     }
     L.nmol = last;
     L.ncomp[ci] = nc - 1;
-    L.nads = last * MC_MAXATOMS;
+    L.nads = last * R.slots;
     L.netCharge -= c.qsum;
     L.U += -Uext + de;
     L.accepted[1] += 1;
@@ -1089,6 +1093,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
 
 // One wavefront per system, blockDim.x / 64 (1, 2 or 4) systems per workgroup: with few systems, one per workgroup
 // spreads them over all compute units.  Dynamic LDS: one WaveScratch per wavefront.
+template <bool CHAIN>
 __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
                                               McSys* __restrict__ Yarr, int nsys, int cycles, int sample)
 {
@@ -1118,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
     const int nsteps = max(R.minStepsPerCycle, L.nmol);
     for (int s = 0; s < nsteps; ++s)
     {
-      gcmcStep(R, S, Y, sys, L, lane, scratch[w]);
+      gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w]);
       waveSyncGlobal();
       L.step += 1;
     }
@@ -1177,12 +1182,12 @@ __global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ 
     const Comp& c = R.comp[Y.comp[m]];
     for (int a = 0; a < c.na; ++a)
     {
-      const int a0 = m * MC_MAXATOMS + a;
+      const int a0 = m * R.slots + a;
       bool o;
       // pairs with molecules of lower index only, so each inter-molecular pair counts once
       double ef = siteExternal(S, 0, S.ax[a0], S.ay[a0], S.az[a0], c.type[a], c.q[a], m, -2, lane, o);
       double ei = 0.0;
-      for (int b = lane; b < m * MC_MAXATOMS; b += 64)
+      for (int b = lane; b < m * R.slots; b += 64)
       {
         const int t = S.at[b];
         if (t < 0) continue;
@@ -1200,9 +1205,9 @@ __global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ 
       double mol[3 * MC_MAXATOMS];
       for (int a = 0; a < c.na; ++a)
       {
-        mol[3 * a] = S.ax[m * MC_MAXATOMS + a];
-        mol[3 * a + 1] = S.ay[m * MC_MAXATOMS + a];
-        mol[3 * a + 2] = S.az[m * MC_MAXATOMS + a];
+        mol[3 * a] = S.ax[m * R.slots + a];
+        mol[3 * a + 1] = S.ay[m * R.slots + a];
+        mol[3 * a + 2] = S.az[m * R.slots + a];
       }
       for (int i = 1; i < c.na; ++i)
       {
@@ -1234,5 +1239,12 @@ __global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ 
   }
   if (lane == 0) out[sys] = e + ef;
 }
+
+template __global__ void k_widom<false>(const McRun*, const SysDev*, const McSys*, int, long long, int, double*,
+                                        double*, double*);
+template __global__ void k_widom<true>(const McRun*, const SysDev*, const McSys*, int, long long, int, double*,
+                                       double*, double*);
+template __global__ void k_gcmc<false>(const McRun*, SysDev*, McSys*, int, int, int);
+template __global__ void k_gcmc<true>(const McRun*, SysDev*, McSys*, int, int, int);
 
 }  // namespace rg

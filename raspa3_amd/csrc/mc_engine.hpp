@@ -87,7 +87,7 @@ struct McRun
   int ncomp;
   int maxMolecules;
   int minStepsPerCycle;
-  int pad;
+  int slots;              // atom slots per molecule in the adsorbate arrays: the largest n_atoms of the components
   double pTranslation, pRotation;  // swap probability = 1 - both, insertion/deletion 50/50
   double maxTranslation, maxRotation;
   long long productionCycles;

@@ -9,7 +9,9 @@
 
 namespace rg
 {
+template <bool CHAIN>
 __global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample);
+template <bool CHAIN>
 __global__ void k_widom(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first, int count,
                         double* outLogW, double* outU, double* outEwald);
 __global__ void k_total_energy(const McRun* Rp, const SysDev* Sarr, const McSys* Yarr, int nsys, double* out);
@@ -124,6 +126,7 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
     }
     C.molFraction = d.mol_fraction / ysum;
     C.fugacity = C.molFraction * p->pressure_pa / pressureConversion;  // fugacity coefficient 1
+    R.slots = std::max(R.slots, C.na);
   }
   return RG_OK;
 }
@@ -139,14 +142,16 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
   if (!ctx || n_systems <= 0 || !systems || !components || !params || !out)
     return fail(ctx, RG_ERR_INVALID, "rg_mc_create: invalid arguments");
   const rg_forcefield* ff = systems[0]->ff;
+  int slots = 1;
+  for (int c = 0; c < n_components; ++c) slots = std::max(slots, components[c].n_atoms);
   for (int s = 0; s < n_systems; ++s)
   {
     if (!systems[s]) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: system %d is null", s);
     if (systems[s]->ff != ff) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: systems must share one force field");
     if (systems[s]->h.nads != 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_create: systems must start empty");
-    if (systems[s]->h.cap < params->max_molecules * MC_MAXATOMS)
+    if (systems[s]->h.cap < params->max_molecules * slots)
       return fail(ctx, RG_ERR_INVALID, "rg_mc_create: system %d capacity below %d atoms", s,
-                  params->max_molecules * MC_MAXATOMS);
+                  params->max_molecules * slots);
   }
   RG_ENTER(ctx);
   rg_mc* mc = new rg_mc();
@@ -288,9 +293,12 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   // are launched eagerly with each candidate and timed.  Every candidate computes the same trajectory (one
   // wavefront per system in every shape), so the choice only affects speed.
   static const int kWpb[3] = {1, 2, 4};
+  bool chain = false;
+  for (int c = 0; c < mc->run.ncomp; ++c) chain = chain || mc->run.comp[c].flexible;
   auto launch = [&](int wpb)
   {
-    hipLaunchKernelGGL(k_gcmc, dim3((nsys + wpb - 1) / wpb), dim3(64 * wpb), wpb * sizeof(WaveScratch), ctx->stream,
+    hipLaunchKernelGGL(chain ? k_gcmc<true> : k_gcmc<false>, dim3((nsys + wpb - 1) / wpb), dim3(64 * wpb),
+                       wpb * sizeof(WaveScratch), ctx->stream,
                        mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
     return hipGetLastError();
   };
@@ -389,7 +397,7 @@ rg_status rg_mc_adsorbates(rg_ctx* ctx, const rg_mc* mc, int system, int capacit
   RG_HIP(ctx, hipMemcpyAsync(&Y, mc->dY + system, sizeof(McSys), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(&S, mc->dS + system, sizeof(SysDev), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const int n = Y.nmol * MC_MAXATOMS;
+  const int n = Y.nmol * mc->run.slots;
   *n_out = n;
   if (n > capacity || n == 0) return n > capacity ? fail(ctx, RG_ERR_INVALID, "capacity below %d", n) : RG_OK;
   std::vector<double> x(n), y(n), z(n);
@@ -455,7 +463,8 @@ rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* c
   if (e == hipSuccess) e = hipMemcpyAsync(dY, &Y, sizeof(McSys), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
   {
-    hipLaunchKernelGGL(k_widom, dim3((count + 3) / 4), dim3(256), 0, ctx->stream, dR, sys->d, dY, 0, first, count,
+    hipLaunchKernelGGL(component->flexible ? k_widom<true> : k_widom<false>, dim3((count + 3) / 4), dim3(256), 0,
+                       ctx->stream, dR, sys->d, dY, 0, first, count,
                        out, out + count, out + 2 * size_t(count));
     e = hipGetLastError();
   }

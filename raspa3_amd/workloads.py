@@ -237,17 +237,19 @@ def forcefield_zeolite(extra=()):
 
 
 def framework_itq29(spec, rep=3):
-    """ITQ-29 (LTA, cubic a = 11.8671 A, 24 Si + 48 O per cell; the reference's test structure) repeated rep^3
-    times, with the force field's charges.  Returns (framework, unit cell)."""
+    """ITQ-29 (LTA, cubic a = 11.8671 A, 24 Si + 48 O per cell; the reference's test structure) repeated rep^3 times
+    (or rep = (nx, ny, nz) times), with the force field's charges.  Returns (framework, unit cell)."""
     with open(os.path.join(DATA_DIR, "itq29.json")) as f:
         d = json.load(f)
     a = d["a"]
     frac = np.array(d["fractional"], float)
     typ = np.array([spec.index(e) for e in d["element"]], np.int32)
-    shifts = np.array([[i, j, k] for i in range(rep) for j in range(rep) for k in range(rep)], float)
+    n = (rep, rep, rep) if np.isscalar(rep) else tuple(rep)
+    shifts = np.array([[i, j, k] for i in range(n[0]) for j in range(n[1]) for k in range(n[2])], float)
     pos = ((frac[None, :, :] + shifts[:, None, :]) * a).reshape(-1, 3)
-    t = np.tile(typ, rep ** 3)
-    return (Framework(f"itq29x{rep}", np.diag([a * rep] * 3), True, pos, t, spec.charge[t].astype(float)),
+    t = np.tile(typ, len(shifts))
+    name = f"itq29x{n[0]}x{n[1]}x{n[2]}"
+    return (Framework(name, np.diag([a * n[0], a * n[1], a * n[2]]), True, pos, t, spec.charge[t].astype(float)),
             np.diag([a, a, a]))
 
 
