@@ -27,7 +27,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "erfcx_table.hpp"
+#include "ewald_tables.hpp"
 #include "rg_internal.hpp"
 
 namespace
@@ -95,33 +95,69 @@ __device__ __forceinline__ double rsq64(double x)
   return y;
 }
 
-// erfcx(x) = exp(x^2) erfc(x) for 0 <= x < 16 from the piecewise Chebyshev interpolant of erfcx_table.hpp (within
-// 3e-16 relative); 0 from 16 on, where erfc(x) and exp(-x^2) are below 1e-111 and every real-space term vanishes to
-// FP64 precision.  Each piece's record sits in LDS (tab, stride kErfcxDeg + 3: midpoint, inverse half-width,
-// coefficients).  The lanes of a wavefront sit within a fraction of an angstrom of each other, so they nearly always
-// share a piece: the loop takes one pass per distinct piece present, and all lanes of a pass read the same record
-// (LDS broadcast).
-__device__ __forceinline__ double erfcxWave(double x, const double* __restrict__ tab)
+// exp(-y) for y >= 0: Cody-Waite reduction exp(-y) = 2^-n exp(r) with n = rint(y / ln 2) and r = n ln 2 - y formed
+// in two FMA steps (ln 2 split into its nearest double and the remainder), |r| <= ln(2)/2; exp(r) from the degree-11
+// Chebyshev-derived polynomial of ewald_tables.hpp (within 2.3e-16 relative); ldexp underflows to 0 for huge y.
+__device__ __forceinline__ double expNeg(double y)
 {
-  const int p = x < 4.0 ? int(x) : (x < 6.0 ? 4 : (x < 8.0 ? 5 : (x < 12.0 ? 6 : (x < 16.0 ? 7 : 8))));
-  double g = 0.0;
-  unsigned long long todo = __ballot(p < rg::kErfcxPieces);
-  while (todo)
-  {
-    const int q = __builtin_amdgcn_readlane(p, __builtin_ctzll(todo));
-    const bool mine = p == q;
-    if (mine)
-    {
-      const double* rec = tab + q * (rg::kErfcxDeg + 3);
-      const double t = (x - rec[0]) * rec[1];
-      double acc = rec[2 + rg::kErfcxDeg];
+  constexpr double kInvLn2 = 1.4426950408889634;
+  constexpr double kLn2Hi = 0.69314718055994529;
+  constexpr double kLn2Lo = 2.3190468138462996e-17;
+  const double n = rint(y * kInvLn2);
+  double r = fma(n, kLn2Hi, -y);
+  r = fma(n, kLn2Lo, r);
+  // Estrin's scheme: pairs c_2k + c_2k+1 r, combined with r^2, then r^4
+  static_assert(rg::kExpDeg == 11, "Estrin tree below is written for degree 11");
+  const double* c = rg::kExpCoef;
+  const double r2 = r * r, r4 = r2 * r2;
+  const double p01 = fma(c[1], r, c[0]), p23 = fma(c[3], r, c[2]), p45 = fma(c[5], r, c[4]);
+  const double p67 = fma(c[7], r, c[6]), p89 = fma(c[9], r, c[8]), pab = fma(c[11], r, c[10]);
+  const double p03 = fma(p23, r2, p01), p47 = fma(p67, r2, p45), p8b = fma(pab, r2, p89);
+  const double p = fma(fma(p8b, r4, p47), r4, p03);
+  return ldexp(p, -static_cast<int>(n));
+}
+
+// LDS image of the erfcx tables of ewald_tables.hpp: the fine pieces (10 coefficients each), then per tail piece
+// {midpoint, inverse half-width, 17 coefficients}
+constexpr int kFineW = rg::kErfcxFineDeg + 1;
+constexpr int kTailW = rg::kErfcxTailDeg + 3;
+constexpr int kErfcxTabLen = rg::kErfcxFinePieces * kFineW + rg::kErfcxTailPieces * kTailW;
+
+__device__ __forceinline__ double erfcxTabEntry(int q)
+{
+  if (q < rg::kErfcxFinePieces * kFineW) return rg::kErfcxFine[q / kFineW][q % kFineW];
+  q -= rg::kErfcxFinePieces * kFineW;
+  const int piece = q / kTailW, k = q % kTailW;
+  return k == 0 ? rg::kErfcxTailMid[piece] : (k == 1 ? rg::kErfcxTailInvHalf[piece] : rg::kErfcxTail[piece][k - 2]);
+}
+
+// erfcx(x) = exp(x^2) erfc(x) for x >= 0 from the piecewise Chebyshev interpolants of ewald_tables.hpp (within
+// 2.3e-16 relative).  On [0, 4) every lane reads the coefficients of its own piece p = floor(8x) with
+// t = 16x - (2p + 1), which is exact.  From 4 on (alpha r >= 4 inside the cutoff, rare) a wave-uniform branch
+// evaluates the tail pieces; from 16 on the result is 0, where erfc(x) and exp(-x^2) are below 1e-111 and every
+// real-space term vanishes to FP64 precision.
+__device__ __forceinline__ double erfcxLds(double x, const double* __restrict__ tab)
+{
+  const int p = min(static_cast<int>(8.0 * x), rg::kErfcxFinePieces - 1);
+  const double t = fma(16.0, x, -static_cast<double>(2 * p + 1));
+  const double* c = tab + p * kFineW;
+  double acc = c[rg::kErfcxFineDeg];
 #pragma unroll
-      for (int j = rg::kErfcxDeg - 1; j >= 0; --j) acc = fma(acc, t, rec[2 + j]);
-      g = acc;
+  for (int j = rg::kErfcxFineDeg - 1; j >= 0; --j) acc = fma(acc, t, c[j]);
+  if (__ballot(x >= 4.0))
+  {
+    if (x >= 4.0)
+    {
+      const int q = x < 6.0 ? 0 : (x < 8.0 ? 1 : (x < 12.0 ? 2 : 3));
+      const double* rec = tab + rg::kErfcxFinePieces * kFineW + q * kTailW;
+      const double tt = (x - rec[0]) * rec[1];
+      double a = rec[2 + rg::kErfcxTailDeg];
+#pragma unroll
+      for (int j = rg::kErfcxTailDeg - 1; j >= 0; --j) a = fma(a, tt, rec[2 + j]);
+      acc = x < rg::kErfcxMax ? a : 0.0;
     }
-    todo &= ~__ballot(mine);
   }
-  return g;
+  return acc;
 }
 
 template <bool TRIC>
@@ -164,7 +200,7 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   __shared__ int rowStart[MAXROWS];
   __shared__ int rowPrefix[MAXROWS + 1];
   __shared__ int waveCount[4];
-  __shared__ __attribute__((aligned(16))) double erfcxTab[COUL ? rg::kErfcxPieces * (rg::kErfcxDeg + 3) : 1];
+  __shared__ __attribute__((aligned(16))) double erfcxTab[COUL ? kErfcxTabLen : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -186,12 +222,7 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   }
   if (COUL)
   {
-    constexpr int W = rg::kErfcxDeg + 3;
-    for (int q = tid; q < rg::kErfcxPieces * W; q += 256)
-    {
-      const int piece = q / W, k = q % W;
-      erfcxTab[q] = k == 0 ? rg::kErfcxMid[piece] : (k == 1 ? rg::kErfcxInvHalf[piece] : rg::kErfcxCoef[piece][k - 2]);
-    }
+    for (int q = tid; q < kErfcxTabLen; q += 256) erfcxTab[q] = erfcxTabEntry(q);
   }
 
   // this lane's grid point
@@ -320,11 +351,20 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     }
     __syncthreads();
 
-    // ---- per-lane pair sums over the staged candidates
+    // ---- per-lane pair sums over the staged candidates; the position and charge of the next candidate are read
+    // while the current one is summed
+    const double2* cb = reinterpret_cast<const double2*>(cand);
+    double2 nx0 = cb[0], nx1 = cb[1];
     for (int c = 0; c < ncand; ++c)
     {
+      const double2 c0 = nx0, c1 = nx1;  // (x, y), (z, q)
+      {
+        const int cn = min(c + 1, ncand - 1);
+        nx0 = cb[4 * cn];
+        nx1 = cb[4 * cn + 1];
+      }
       const Img& b = cand[c];
-      const double dx = px - b.x, dy = py - b.y, dz = pz - b.z;
+      const double dx = px - c0.x, dy = py - c0.y, dz = pz - c1.x;
       double rr = dx * dx + dy * dy + dz * dz;
       if (rr < A.rc2)
       {
@@ -365,24 +405,25 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
             }
           }
         }
-        if (COUL && b.q != 0.0)
+        // a Coulomb-only launch stages charged images only
+        if (COUL && (NLJ == 0 || c1.y != 0.0))
         {
           // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 with a unit probe charge;
-          // b.q already carries the Coulomb conversion factor.
+          // the staged charge already carries the Coulomb conversion factor.
           const double ir = rsq64(rr);
           const double r = rr * ir;
           const double ir2 = ir * ir;
           const double ir3 = ir2 * ir;
           const double ir5 = ir3 * ir2;
           const double ir7 = ir5 * ir2;
-          const double ex = exp(-A.alpha2 * rr);
-          const double ec = ex * erfcxWave(A.alpha * r, erfcxTab);
-          const double etr = A.k2a * r * ex;
           const double ar2 = A.alpha2 * rr;
-          const double e = b.q * ec * ir;
-          const double f1 = -b.q * (etr + ec) * ir3;
-          const double f2 = b.q * fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir5;
-          const double f3 = -b.q * fma(etr, fma(ar2, fma(4.0, ar2, 10.0), 15.0), 15.0 * ec) * ir7;
+          const double qex = c1.y * expNeg(ar2);               // q exp(-alpha^2 r^2)
+          const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);  // q erfc(alpha r)
+          const double etr = A.k2a * r * qex;                 // q 2 alpha / sqrt(pi) r exp(-alpha^2 r^2)
+          const double e = ec * ir;
+          const double f1 = -(etr + ec) * ir3;
+          const double f2 = fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir5;
+          const double f3 = -fma(etr, fma(ar2, fma(4.0, ar2, 10.0), 15.0), 15.0 * ec) * ir7;
           acc[NG - 1].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
         }
       }

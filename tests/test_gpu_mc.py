@@ -263,3 +263,32 @@ def test_gcmc_hexane_replays_oracle_trajectory(gpu_ctx):
         e = mc.state(sidx)["energy"]
         assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
     mc.close()
+
+
+def test_gcmc_uneven_batch_replays_oracle(gpu_ctx):
+    """Systems of different sizes in one launch: n-hexane (flexible) + CO2 (Ewald kmax 6) at 1e5 Pa and mole
+    fractions 0.1 / 0.9 in ITQ-29 3x3x3 (1944 sites) and 3x3x4 (2592 sites), interpolation grids shared from the
+    unit cell.  Each system replays its own oracle trajectory for 3 + 8 cycles, and its running energy equals the
+    recomputed total."""
+    print("case: gcmc uneven batch (hexane + CO2, 3x3x3 and 3x3x4)", flush=True)
+    spec = w.forcefield_hexane_zeolite()
+    types = [spec.index(n) for n in ("CH3", "CH2", "C_co2", "O_co2")]
+    fw3, unit, ff, f3, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    fw4, _ = w.framework_itq29(spec, (3, 3, 4))
+    f4 = ra.Framework(gpu_ctx, fw4)
+    comps = [w.hexane_component(spec, 0.3, 0.1), w.co2_reference(spec, 0.9)]
+    params = w.mc_params(pressure_pa=1e5, seed=19, max_molecules=96, production_cycles=8)
+    fws = [fw3, fw4]
+    systems = [ra.System(gpu_ctx, ff, fw.cell, True, framework=fo, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                         capacity=6 * 96) for fw, fo in zip(fws, [f3, f4])]
+    mc = ra.Mc(gpu_ctx, systems, comps, params)
+    mc.run(3, sample=False, graph=True)
+    mc.run(8, sample=True, graph=True)
+    for sidx, fw in enumerate(fws):
+        g = compare_with_oracle(mc, sidx, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, unit, (3, 8))
+        assert g["accepted"][0] > 0 and g["n_per_component"][0] > 0 and g["n_per_component"][1] > 0
+    tot = mc.total_energy()
+    for sidx in range(2):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
