@@ -45,6 +45,12 @@ FLOP_LJ_TRIC = 79
 FLOP_EWALD_TRIC = 86
 GRID_BYTES_PER_SITE = 512      # 8 corner chunks of 64 B per tricubic interpolation
 MLP_FLOP_PER_SAMPLE = 1.08e6   # forward 2 * 180.5k + backward 4 * 180.5k (GEMM work of a training sample)
+# HBM bytes per launch of the Ewald grid kernel on the headline workload, from two rocprofv3 --pmc runs of
+# "bench.py --grids-only --steps 1 --warmup 1 --no-cpu-baseline" (FETCH_SIZE 25,742 KiB, WRITE_SIZE 2,910,675 KiB per
+# dispatch), each scaled by what the same counter reads for a 1 GiB device-to-device copy ("bench.py
+# --copy-calibration 1024": FETCH_SIZE 524,300 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0).  The algorithmic
+# bytes are the 64 B written per point (2.98e9) and the 1.2 MB image list.
+EWALD_GRID_HBM_BYTES = 25742 * 1024 * 2.0 + 2910675 * 1024 * 1.0
 
 
 def log(*a):
@@ -160,7 +166,10 @@ def bench_grids(ctx, args):
     achieved = flop_q / (q_ms / 1000.0) / 1e12
     lj_achieved = flop_lj / (lj_ms / 1000.0) / 1e12
     roof = dict(bound="valu", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=None,
+                frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=EWALD_GRID_HBM_BYTES,
+                traffic_algorithmic=float(npts * 64), traffic_note="HBM bytes per Ewald launch (rocprofv3 "
+                "FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy); the kernel writes 64 B per point and is bound "
+                "by FP64 VALU, not HBM",
                 kernel="k_grid_build<0,true,true> (Ewald real-space grid, triclinic), the dominant kernel of the step",
                 flop_per_launch=float(flop_q), kernel_ms=round(q_ms, 4),
                 incutoff_pairs_per_point=dict(lj_active=round(n_lj, 2), charged=round(n_all, 2)),
@@ -488,6 +497,9 @@ def main():
     ap.add_argument("--uneven-systems", type=int, default=64)
     ap.add_argument("--uneven-equilibration", type=int, default=300)
     ap.add_argument("--uneven-production", type=int, default=100)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 3 times (the reference kernel that scales the HBM "
+                         "byte counters of the traffic figure) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build path shards over "
@@ -495,6 +507,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes=n, reads_per_launch=n, writes_per_launch=n)), flush=True)
+        return
 
     import raspa3_amd as ra
 
