@@ -503,9 +503,11 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// flexible linear chains (united-atom alkanes): bond geometry sampled exactly from the bonded Boltzmann factors
-// (Frenkel & Smit, Understanding Molecular Simulation, 2nd ed., sec. 13.3), Rosenbluth weights over the external
-// and intra-molecular non-bonded energies.  Lane j builds trial j of a bead; its rejection samplers draw from the
+// flexible linear chains (united-atom alkanes), the counterpart of CBMC::growFlexibleMoleculeChainInsertion
+// (src/raspakit/cbmc/flexible_molecules/insertion.cpp:32-126) and retraceFlexibleMoleculeChainDeletion
+// (src/raspakit/cbmc/flexible_molecules/deletion.cpp:31-104): bond geometry sampled exactly from the bonded
+// Boltzmann factors (Frenkel & Smit, Understanding Molecular Simulation, 2nd ed., sec. 13.3), Rosenbluth weights
+// over the external and intra-molecular non-bonded energies.  Lane j builds trial j of a bead; its rejection samplers draw from the
 // Philox slots SLOT_CHAIN + ((bead * 64 + trial) * 4 + kind) * MC_MAXREJ + attempt, the bead's selection from
 // SLOT_CHAIN - 16 + bead.
 // ---------------------------------------------------------------------------------------------------------------
