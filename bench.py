@@ -24,6 +24,8 @@ Second half of the metric, in the same JSON line:
   their standard errors (over systems), and whether they agree within 2 sigma.
 * mlp: training throughput of the proposal network (BF16 MFMA) on 2^21 points of the configuration-1 CO2 grids at
   300 K, samples/s and MFMA TFLOP/s at 1.08 MFLOP per training sample.
+* grid_check: the test the reference runs after building grids (1e5 random points per LJ grid, Boltzmann-weighted
+  relative errors of the interpolated energy and gradient, and of the Ewald grid scaled by the probe charge).
 * gcmc_uneven: batched CBMC over 64 systems of uneven size and loading (ITQ-29 supercells of 1944-7200 sites),
   flexible n-hexane chains and rigid CO2 with the Ewald difference at kmax 8, one launch per cycle.
 """
@@ -179,6 +181,20 @@ def bench_grids(ctx, args):
                                            flop_per_launch=float(flop_lj))),
                 step_achieved=round((flop_q + flop_lj) / ((lj_ms + q_ms) / 1000.0) / 1e12, 3))
     state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, dims=dims)
+    # the test the reference runs after building (1e5 random points per LJ grid, with the Ewald grid scaled by the
+    # probe's charge); outside the timed region
+    check = {}
+    t = time.perf_counter()
+    for g, name in ((grids[0], "C_co2"), (grids[1], "O_co2")):
+        pr = spec.index(name)
+        c = ra.grid_check(ctx, ff, f, g, pr, grids[2], float(spec.charge[pr]), 300.0, n_points=100000, seed=7)
+        check[name] = dict(lj_energy_relative_error=c["lennard_jones"]["energy_relative_error"],
+                           lj_gradient_relative_error=c["lennard_jones"]["gradient_relative_error"].tolist(),
+                           ewald_energy_relative_error=c["coulomb"]["energy_relative_error"],
+                           ewald_points=c["n_coulomb_points"])
+    check["seconds"] = round(time.perf_counter() - t, 4)
+    state["grid_check"] = check
+    log(f"grid check (1e5 points per LJ grid): {json.dumps(check)}")
     return value, ms_per_step, npts, roof, state
 
 
@@ -524,7 +540,7 @@ def main():
     torch.cuda.set_device(0)
     ctx = ra.Context(0)
     value, ms_per_step, npts, roof, g1 = bench_grids(ctx, args)
-    extra = {}
+    extra = {"grid_check": g1["grid_check"]}
     if not args.grids_only:
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)

@@ -146,6 +146,35 @@ rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count
 rg_status rg_grid_interpolate(rg_ctx* ctx, const rg_grid* grid, int n, const double* pos, double* energy,
                               double* gradient);
 
+/* Test of a built grid on random points, as System::createFrameworkInterpolationGrids does after building
+ * (src/raspakit/system_interpolation_grids.cpp:240-840, numberOfGridTestPoints = 1e5 by default): the tricubic
+ * interpolation of the Lennard-Jones grid of pseudo-atom 'probe' (and, when coulomb_grid is non-null, of the
+ * unit-charge Ewald real-space grid scaled by probe_charge) against the analytic sums of
+ * Interactions::calculateTricubicFractionalAtPosition at the same points.  Points are uniform in the unit cell:
+ * frac (n_points * 3 fractional coordinates) when non-null, else drawn from a Philox4x32-10 stream keyed by seed.
+ * Every point is weighted by exp(-beta E_LJ) with beta = 1/(k_B T) in inverse internal energy units and E_LJ in K,
+ * as the reference computes it; the Ewald statistics take the points whose E_LJ (K) is below the overlap criterion.
+ * Energies in K, gradients in K/A (Cartesian). */
+typedef struct rg_grid_check_stats
+{
+  double weight;                      /* sum of the Boltzmann weights */
+  double energy_table, energy_full;   /* Boltzmann averages, interpolated and analytic */
+  double energy_relative_error;       /* sqrt(sum w (full - table)^2 / sum w full^2) */
+  double gradient_table[3], gradient_full[3], gradient_relative_error[3];
+} rg_grid_check_stats;
+
+typedef struct rg_grid_check_t
+{
+  long long n_points;
+  rg_grid_check_stats lennard_jones;
+  long long n_coulomb_points;
+  rg_grid_check_stats coulomb;        /* zero when coulomb_grid is NULL */
+} rg_grid_check_t;
+
+rg_status rg_grid_check(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, const rg_grid* lj_grid, int probe,
+                        const rg_grid* coulomb_grid, double probe_charge, double temperature, int n_points,
+                        const double* frac, unsigned long long seed, rg_grid_check_t* out);
+
 /* ---------------------------------------------------------------------------------------------
  * Systems: what CBMC::GrowContext (src/raspakit/cbmc/cbmc_growth_context.ixx:15-29) gives the trial-energy code —
  * box, force field, framework atoms or interpolation grids, adsorbate atoms — plus the Ewald state

@@ -801,6 +801,113 @@ void orc_interpolate(const double* cell9, int rect, int nx, int ny, int nz, cons
   for (int q = 0; q < n; ++q) interpolateOne(unit, np, grid, pos + 3 * q, e + q, grad ? grad + 3 * q : nullptr);
 }
 
+// The grid test of System::createFrameworkInterpolationGrids (src/raspakit/system_interpolation_grids.cpp:240-840,
+// tricubic branch) at n points given as fractional coordinates of the unit cell: the interpolated energy and
+// gradient of the LJ grid (and of the unit-charge Ewald grid times probe_charge, when coul_grid is non-null) against
+// calculateTricubicFractionalAtPosition over the supercell, its fractional gradient turned Cartesian with the
+// transposed inverse cell (:302-305); weights exp(-beta E_LJ) with beta in inverse internal units and E_LJ in K
+// (:365, :545); the Ewald part over the points with E_LJ (K) below the overlap criterion (:420).
+// out[28] = {n, LJ stats[13], n_coulomb, Coulomb stats[13]}; stats = {sum w, <E> table, <E> full, relative error,
+// <grad> table[3], <grad> full[3], relative error[3]}, energies in K, gradients in K/A.
+void orc_grid_check(const double* cell9, int rect, int nx, int ny, int nz, int natoms, const double* pos,
+                    const int* type, const double* charge, int ntypes, const double* pair_eps, const double* pair_sigma,
+                    const double* pair_shift, const int* pair_lj, int probe, double rc_vdw, double rc_supercell,
+                    double alpha, double overlap, const double* lj_grid, const double* coul_grid, double probe_charge,
+                    double temperature, int n, const double* frac, double* out)
+{
+  Box unit = makeBox(cell9, rect);
+  double w[3];
+  perpendicularWidths(unit.cell, w);
+  int nc[3];
+  for (int a = 0; a < 3; ++a) nc[a] = static_cast<int>(std::ceil(2.0 * rc_supercell / w[a]));
+  std::vector<double> spos;
+  std::vector<int> stype;
+  std::vector<double> sq;
+  AtomsView av{natoms, pos, type, charge};
+  makeSuperCell(unit.cell, av, nc, spos, stype, sq);
+  Box super = unit;
+  for (int col = 0; col < 3; ++col)
+    for (int r = 0; r < 3; ++r) super.cell.c[col][r] = unit.cell.c[col][r] * nc[col];
+  super.inv = inverse(super.cell);
+  FF ff{ntypes, pair_eps, pair_sigma, pair_shift, pair_lj};
+  const int np[3] = {nx, ny, nz};
+  const int ns = static_cast<int>(stype.size());
+  const double beta = 1.0 / (U.kB * temperature);
+  const double K = U.energyToKelvin;
+  struct Sums
+  {
+    double w = 0, t = 0, f = 0, d2 = 0, f2 = 0, gt[3] = {0, 0, 0}, gf[3] = {0, 0, 0}, gd2[3] = {0, 0, 0},
+           gf2[3] = {0, 0, 0};
+  } sl, sc;
+  auto add = [](Sums& S, double wt, double et, double ef, const double* gt, const double* gf)
+  {
+    S.w += wt;
+    S.t += et * wt;
+    S.f += ef * wt;
+    S.d2 += (ef - et) * (ef - et) * wt;
+    S.f2 += ef * ef * wt;
+    for (int a = 0; a < 3; ++a)
+    {
+      S.gt[a] += gt[a] * wt;
+      S.gf[a] += gf[a] * wt;
+      S.gd2[a] += (gf[a] - gt[a]) * (gf[a] - gt[a]) * wt;
+      S.gf2[a] += gf[a] * gf[a] * wt;
+    }
+  };
+  // Cartesian gradient from fractional derivatives: inverseCell^T (v1, v2, v3)
+  auto cart = [&](const double v[8], double scale, double g[3])
+  {
+    for (int c = 0; c < 3; ++c)
+      g[c] = scale * (unit.inv.c[c][0] * v[1] + unit.inv.c[c][1] * v[2] + unit.inv.c[c][2] * v[3]);
+  };
+  long long nq = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    double p[3];
+    mulv(unit.cell, frac + 3 * i, p);
+    double et, gt[3], v[8], gf[3];
+    interpolateOne(unit, np, lj_grid, p, &et, gt);
+    et *= K;
+    for (double& x : gt) x *= K;
+    tricubicFractionalAtPosition(0, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
+                                 sq.data(), v);
+    const double ef = v[0] * K;
+    cart(v, K, gf);
+    const double wt = std::exp(-beta * ef);
+    add(sl, wt, et, ef, gt, gf);
+    if (coul_grid && ef < overlap)
+    {
+      double ct, cgt[3], cv[8], cgf[3];
+      interpolateOne(unit, np, coul_grid, p, &ct, cgt);
+      ct *= probe_charge * K;
+      for (double& x : cgt) x *= probe_charge * K;
+      tricubicFractionalAtPosition(1, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
+                                   sq.data(), cv);
+      cart(cv, probe_charge * K, cgf);
+      add(sc, wt, ct, cv[0] * probe_charge * K, cgt, cgf);
+      ++nq;
+    }
+  }
+  auto store = [](const Sums& S, double* o)
+  {
+    o[0] = S.w;
+    o[1] = S.t / S.w;
+    o[2] = S.f / S.w;
+    o[3] = std::sqrt(S.d2 / S.f2);
+    for (int a = 0; a < 3; ++a)
+    {
+      o[4 + a] = S.gt[a] / S.w;
+      o[7 + a] = S.gf[a] / S.w;
+      o[10 + a] = std::sqrt(S.gd2[a] / S.gf2[a]);
+    }
+  };
+  for (int k = 0; k < 28; ++k) out[k] = 0.0;
+  out[0] = n;
+  store(sl, out + 1);
+  out[14] = double(nq);
+  if (nq > 0) store(sc, out + 15);
+}
+
 // The 64x64 tricubic coefficient matrix (row-major), for tests.
 void orc_tricubic_matrix(double* out)
 {

@@ -287,6 +287,29 @@ def build_grids(ctx, ff, fw, specs, sync=True):
     L.check(fn(ctx.h, ff.h, fw.h, n, L.i(types), L.i(probes), L.d(cuts), arr), ctx.h, "rg_grid_build_many")
 
 
+def grid_check(ctx, ff, fw, lj_grid, probe, coulomb_grid=None, probe_charge=0.0, temperature=300.0,
+               n_points=100000, frac=None, seed=1):
+    """rg_grid_check: the reference's test of a built grid on random points (Boltzmann-weighted averages of the
+    interpolated and analytic energy and gradient, relative errors).  Returns dict(n_points, lennard_jones,
+    n_coulomb_points, coulomb)."""
+    out = L.GridCheck()
+    f = None if frac is None else L.f64(frac).reshape(-1, 3)
+    n = n_points if f is None else len(f)
+    L.check(lib().rg_grid_check(ctx.h, ff.h, fw.h, lj_grid.h, int(probe),
+                                None if coulomb_grid is None else coulomb_grid.h, float(probe_charge),
+                                float(temperature), int(n), L.d(f), C.c_ulonglong(seed), C.byref(out)), ctx.h,
+            "rg_grid_check")
+
+    def stats(x):
+        return dict(weight=x.weight, energy_table=x.energy_table, energy_full=x.energy_full,
+                    energy_relative_error=x.energy_relative_error, gradient_table=np.array(list(x.gradient_table)),
+                    gradient_full=np.array(list(x.gradient_full)),
+                    gradient_relative_error=np.array(list(x.gradient_relative_error)))
+
+    return dict(n_points=out.n_points, lennard_jones=stats(out.lennard_jones), n_coulomb_points=out.n_coulomb_points,
+                coulomb=stats(out.coulomb))
+
+
 def component_desc(d):
     """rg_component_desc from a dict(n_atoms, first_bead, position (n,3), type, charge, mol_fraction[, flexible,
     bond_length, bend_theta0_deg, bend_k_K, torsion_K, log_ideal_rosenbluth])."""

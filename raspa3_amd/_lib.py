@@ -126,7 +126,20 @@ class McState(C.Structure):
                 ("block_nn", ((C.c_double * 2) * 2) * 5)]
 
 
+class GridCheckStats(C.Structure):
+    _fields_ = [("weight", C.c_double), ("energy_table", C.c_double), ("energy_full", C.c_double),
+                ("energy_relative_error", C.c_double), ("gradient_table", C.c_double * 3),
+                ("gradient_full", C.c_double * 3), ("gradient_relative_error", C.c_double * 3)]
+
+
+class GridCheck(C.Structure):
+    _fields_ = [("n_points", C.c_longlong), ("lennard_jones", GridCheckStats), ("n_coulomb_points", C.c_longlong),
+                ("coulomb", GridCheckStats)]
+
+
 SIGNATURES.update({
+    "rg_grid_check": (C.c_int, [vp, vp, vp, vp, C.c_int, vp, C.c_double, C.c_double, C.c_int, dp, C.c_ulonglong,
+                                C.POINTER(GridCheck)]),
     "rg_widom": (C.c_int, [vp, vp, C.POINTER(ComponentDesc), C.POINTER(McParams), C.c_longlong, C.c_int, dp, dp,
                            dp]),
     "rg_mc_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(ComponentDesc), C.POINTER(McParams),

@@ -28,6 +28,7 @@
 #include <cstring>
 
 #include "ewald_tables.hpp"
+#include "mc_device.hpp"
 #include "rg_internal.hpp"
 
 namespace
@@ -583,6 +584,42 @@ rg_status buildImages(rg_framework* fw, double rc, const std::vector<int>& typeR
   return RG_OK;
 }
 
+// Local rows for the framework's distinct pseudo-atom types (typeRow[t] = row of type t, rowType[row] = type).
+rg_status frameworkTypeRows(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, std::vector<int>& typeRow,
+                            std::vector<int>& rowType, const char* who)
+{
+  typeRow.assign(ff->n, 0);
+  rowType.clear();
+  std::vector<int> seen(ff->n, -1);
+  for (int t : fw->type)
+  {
+    if (t < 0 || t >= ff->n) return rg::fail(ctx, RG_ERR_INVALID, "%s: framework type out of range", who);
+    if (seen[t] < 0)
+    {
+      seen[t] = static_cast<int>(rowType.size());
+      rowType.push_back(t);
+    }
+  }
+  if (rowType.size() > size_t(MAXT))
+    return rg::fail(ctx, RG_ERR_UNSUPPORTED, "%s: more than %d framework pseudo-atom types", who, MAXT);
+  for (int t = 0; t < ff->n; ++t) typeRow[t] = seen[t] < 0 ? 0 : seen[t];
+  return RG_OK;
+}
+
+// Lennard-Jones constants of the probe against each framework type row
+LJPar ljPar(const rg_forcefield* ff, int probe, int type)
+{
+  const size_t k = size_t(probe) * ff->n + type;
+  LJPar P{};
+  P.e4 = 4.0 * ff->eps[k];
+  P.s2 = ff->sigma[k] * ff->sigma[k];
+  P.sh = ff->shift[k];
+  P.c1 = -6.0 * P.e4;
+  P.c2 = 24.0 * P.e4;
+  P.c3 = -96.0 * P.e4;
+  return P;
+}
+
 rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fwc, int n_grids,
                         const rg_grid_type* types, const int* probes, const double* cutoffs, rg_grid* const* grids,
                         bool sync)
@@ -614,22 +651,10 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
   }
   if (g0->n[0] < 2 || g0->n[1] < 2 || g0->n[2] < 2)
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: at least 2 points per axis");
-  for (int t : fw->type)
-    if (t < 0 || t >= ff->n) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: framework type out of range");
-
-  // local rows for the framework's distinct pseudo-atom types
-  std::vector<int> typeRow(ff->n, 0), rowType;
+  std::vector<int> typeRow, rowType;
   {
-    std::vector<int> seen(ff->n, -1);
-    for (int t : fw->type)
-      if (seen[t] < 0)
-      {
-        seen[t] = static_cast<int>(rowType.size());
-        rowType.push_back(t);
-      }
-    if (rowType.size() > size_t(MAXT))
-      return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_build: more than %d framework pseudo-atom types", MAXT);
-    for (int t = 0; t < ff->n; ++t) typeRow[t] = seen[t] < 0 ? 0 : seen[t];
+    const rg_status st = frameworkTypeRows(ctx, ff, fw, typeRow, rowType, "rg_grid_build");
+    if (st != RG_OK) return st;
   }
   RG_ENTER(ctx);
   const int* n = g0->n;
@@ -726,19 +751,7 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
       for (int q = 0; q < L.nlj; ++q)
       {
         const int g = lj[li + q];
-        const int ta = probes[g];
-        for (size_t row = 0; row < rowType.size(); ++row)
-        {
-          const size_t k = size_t(ta) * ff->n + rowType[row];
-          LJPar P{};
-          P.e4 = 4.0 * ff->eps[k];
-          P.s2 = ff->sigma[k] * ff->sigma[k];
-          P.sh = ff->shift[k];
-          P.c1 = -6.0 * P.e4;
-          P.c2 = 24.0 * P.e4;
-          P.c3 = -96.0 * P.e4;
-          table[t0 + q * MAXT + row] = P;
-        }
+        for (size_t row = 0; row < rowType.size(); ++row) table[t0 + q * MAXT + row] = ljPar(ff, probes[g], rowType[row]);
         L.out[q] = grids[g]->d;
       }
       if (L.coul) L.out[L.nlj] = grids[coul[0]]->d;
@@ -773,6 +786,243 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
   return RG_OK;
 }
 
+
+// ---- grid check: the test of System::createFrameworkInterpolationGrids on random points
+// (src/raspakit/system_interpolation_grids.cpp:240-840).  Per point the analytic Lennard-Jones and unit-charge Ewald
+// real-space energies and Cartesian gradients (sum f1 dr over the in-cutoff images, the same pair terms as the
+// builder) are compared with the grids' tricubic interpolation.
+struct CheckArgs
+{
+  const Img* img;
+  int n_images;
+  const LJPar* par;  // [MAXT] rows of the probe
+  const double* pos;  // n * 3, Cartesian, inside the unit cell
+  int n;
+  int coul;
+  double rc2, alpha, alpha2, k2a, coulomb;
+  double* out;  // n * 8: E_lj, dE_lj/dr (3), E_q, dE_q/dr (3) for a unit probe charge; internal units
+};
+
+__global__ __launch_bounds__(256) void k_grid_check(CheckArgs A)
+{
+  __shared__ __attribute__((aligned(16))) double erfcxTab[kErfcxTabLen];
+  for (int q = threadIdx.x; q < kErfcxTabLen; q += 256) erfcxTab[q] = erfcxTabEntry(q);
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool live = i < A.n;
+  const double px = live ? A.pos[3 * i] : 0.0, py = live ? A.pos[3 * i + 1] : 0.0, pz = live ? A.pos[3 * i + 2] : 0.0;
+  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, eq = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0;
+  for (int j = 0; j < A.n_images; ++j)  // uniform over the wavefront: every lane reads the same image
+  {
+    const Img& b = A.img[j];
+    const double dx = px - b.x, dy = py - b.y, dz = pz - b.z;
+    double rr = dx * dx + dy * dy + dz * dz;
+    if (live && rr < A.rc2)
+    {
+      rr = fmax(rr, 0.1);  // src/raspakit/interactions/framework_molecule_grid.cpp:200
+      const LJPar& P = A.par[b.type];
+      if (P.e4 != 0.0)
+      {
+        // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46 (energy and f1)
+        const double ir2 = rcp64(rr);
+        const double x = P.s2 * ir2;
+        const double t3 = x * x * x;
+        const double t6 = t3 * t3;
+        e += fma(P.e4, t6 - t3, -P.sh);
+        const double f1 = P.c1 * fma(2.0, t6, -t3) * ir2;
+        g0 = fma(f1, dx, g0);
+        g1 = fma(f1, dy, g1);
+        g2 = fma(f1, dz, g2);
+      }
+      if (A.coul && b.q != 0.0)
+      {
+        // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 (energy and f1)
+        const double ir = rsq64(rr);
+        const double r = rr * ir;
+        const double q = b.q * A.coulomb;
+        const double qex = q * expNeg(A.alpha2 * rr);
+        const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);
+        const double etr = A.k2a * r * qex;
+        eq += ec * ir;
+        const double f1 = -(etr + ec) * ir * ir * ir;
+        h0 = fma(f1, dx, h0);
+        h1 = fma(f1, dy, h1);
+        h2 = fma(f1, dz, h2);
+      }
+    }
+  }
+  if (!live) return;
+  double* o = A.out + 8 * size_t(i);
+  o[0] = e;
+  o[1] = g0;
+  o[2] = g1;
+  o[3] = g2;
+  o[4] = eq;
+  o[5] = h0;
+  o[6] = h1;
+  o[7] = h2;
+}
+
+// Boltzmann-weighted comparison of 'full' (analytic) and 'table' (interpolated) values
+struct CheckSums
+{
+  double w = 0, table = 0, full = 0, diff2 = 0, full2 = 0;
+  double gt[3] = {0, 0, 0}, gf[3] = {0, 0, 0}, gd2[3] = {0, 0, 0}, gf2[3] = {0, 0, 0};
+  void add(double wt, double et, double ef, const double* gtab, const double* gfull)
+  {
+    w += wt;
+    table += et * wt;
+    full += ef * wt;
+    diff2 += (ef - et) * (ef - et) * wt;
+    full2 += ef * ef * wt;
+    for (int a = 0; a < 3; ++a)
+    {
+      gt[a] += gtab[a] * wt;
+      gf[a] += gfull[a] * wt;
+      gd2[a] += (gfull[a] - gtab[a]) * (gfull[a] - gtab[a]) * wt;
+      gf2[a] += gfull[a] * gfull[a] * wt;
+    }
+  }
+  void store(rg_grid_check_stats* s) const
+  {
+    s->weight = w;
+    s->energy_table = table / w;
+    s->energy_full = full / w;
+    s->energy_relative_error = std::sqrt(diff2 / full2);
+    for (int a = 0; a < 3; ++a)
+    {
+      s->gradient_table[a] = gt[a] / w;
+      s->gradient_full[a] = gf[a] / w;
+      s->gradient_relative_error[a] = std::sqrt(gd2[a] / gf2[a]);
+    }
+  }
+};
+
+rg_status gridCheck(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fwc, const rg_grid* lj, int probe,
+                    const rg_grid* coul, double probeCharge, double temperature, int n, const double* frac,
+                    unsigned long long seed, rg_grid_check_t* out)
+{
+  const char* who = "rg_grid_check";
+  if (!ctx || !ff || !fwc || !lj || !out || n <= 0 || temperature <= 0.0)
+    return rg::fail(ctx, RG_ERR_INVALID, "%s: invalid arguments", who);
+  if (probe < 0 || probe >= ff->n) return rg::fail(ctx, RG_ERR_INVALID, "%s: probe pseudo-atom out of range", who);
+  rg_framework* fw = const_cast<rg_framework*>(fwc);
+  for (const rg_grid* g : {lj, coul})
+    if (g && std::memcmp(g->cell, fw->cell, sizeof(g->cell)) != 0)
+      return rg::fail(ctx, RG_ERR_INVALID, "%s: grid cell differs from the framework unit cell", who);
+  std::vector<int> typeRow, rowType;
+  {
+    const rg_status st = frameworkTypeRows(ctx, ff, fw, typeRow, rowType, who);
+    if (st != RG_OK) return st;
+  }
+  RG_ENTER(ctx);
+  const double rc = ff->rc_framework;
+  if (fw->images.rc != rc)
+  {
+    const rg_status st = buildImages(fw, rc, typeRow, 2.0);
+    if (st != RG_OK) return st;
+  }
+  // test points: uniform in the unit cell (the caller's fractional coordinates, or a Philox4x32-10 stream)
+  std::vector<double> pos(3 * size_t(n));
+  for (int i = 0; i < n; ++i)
+  {
+    double s[3];
+    if (frac)
+      for (int a = 0; a < 3; ++a) s[a] = frac[3 * size_t(i) + a];
+    else
+    {
+      double u3;
+      rgd::uniform2(seed, uint32_t(i), 0x67726964u, 0u, 0u, s[0], s[1]);
+      rgd::uniform2(seed, uint32_t(i), 0x67726964u, 1u, 0u, s[2], u3);
+    }
+    for (int c = 0; c < 3; ++c)
+      pos[3 * size_t(i) + c] = fw->cell[c] * s[0] + fw->cell[3 + c] * s[1] + fw->cell[6 + c] * s[2];
+  }
+  std::vector<double> eLJ(n), gLJ(3 * size_t(n)), eQ, gQ, full(8 * size_t(n));
+  {
+    const rg_status st = rg_grid_interpolate(ctx, lj, n, pos.data(), eLJ.data(), gLJ.data());
+    if (st != RG_OK) return st;
+  }
+  if (coul)
+  {
+    eQ.resize(n);
+    gQ.resize(3 * size_t(n));
+    const rg_status st = rg_grid_interpolate(ctx, coul, n, pos.data(), eQ.data(), gQ.data());
+    if (st != RG_OK) return st;
+  }
+  std::vector<LJPar> par(MAXT);
+  for (size_t row = 0; row < rowType.size(); ++row) par[row] = ljPar(ff, probe, rowType[row]);
+  void *dPar = nullptr, *dPos = nullptr, *dOut = nullptr;
+  auto release = [&]
+  {
+    hipStreamSynchronize(ctx->stream);
+    if (dPar) hipFree(dPar);
+    if (dPos) hipFree(dPos);
+    if (dOut) hipFree(dOut);
+  };
+  auto run = [&]() -> rg_status
+  {
+    RG_HIP(ctx, hipMalloc(&dPar, MAXT * sizeof(LJPar)));
+    RG_HIP(ctx, hipMalloc(&dPos, pos.size() * sizeof(double)));
+    RG_HIP(ctx, hipMalloc(&dOut, full.size() * sizeof(double)));
+    RG_HIP(ctx, hipMemcpyAsync(dPar, par.data(), MAXT * sizeof(LJPar), hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(ctx, hipMemcpyAsync(dPos, pos.data(), pos.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CheckArgs A{};
+    A.img = static_cast<const Img*>(fw->images.d_img);
+    A.n_images = fw->images.n_images;
+    A.par = static_cast<const LJPar*>(dPar);
+    A.pos = static_cast<const double*>(dPos);
+    A.n = n;
+    A.coul = coul ? 1 : 0;
+    A.rc2 = rc * rc;
+    A.alpha = ff->alpha;
+    A.alpha2 = ff->alpha * ff->alpha;
+    A.k2a = 2.0 * ff->alpha / std::sqrt(M_PI);
+    A.coulomb = rg::units().coulomb;
+    A.out = static_cast<double*>(dOut);
+    hipLaunchKernelGGL(k_grid_check, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, A);
+    RG_HIP(ctx, hipGetLastError());
+    RG_HIP(ctx, hipMemcpyAsync(full.data(), dOut, full.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RG_OK;
+  };
+  const rg_status st = run();
+  release();
+  if (st != RG_OK) return st;
+
+  // Statistics as the reference accumulates them (energies and gradients in K and K/A): every point is weighted
+  // by exp(-beta E_LJ) with beta = 1 / (k_B T) in inverse internal energy units and E_LJ, the analytic
+  // Lennard-Jones energy, in Kelvin (system_interpolation_grids.cpp:365, :545); the Ewald comparison, scaled by the
+  // probe's charge, takes the points whose analytic E_LJ (K) is below the overlap criterion (:420).
+  const rg::Units& U = rg::units();
+  const double beta = 1.0 / (U.kB * temperature);
+  const double K = U.energyToKelvin;
+  CheckSums sLJ, sQ;
+  long long nQ = 0;
+  for (int i = 0; i < n; ++i)
+  {
+    const double* f = &full[8 * size_t(i)];
+    const double ef = f[0] * K, et = eLJ[i] * K;
+    const double gf[3] = {f[1] * K, f[2] * K, f[3] * K};
+    const double gt[3] = {gLJ[3 * size_t(i)] * K, gLJ[3 * size_t(i) + 1] * K, gLJ[3 * size_t(i) + 2] * K};
+    const double w = std::exp(-beta * ef);
+    sLJ.add(w, et, ef, gt, gf);
+    if (coul && ef < ff->overlap)
+    {
+      const double qK = probeCharge * K;
+      const double qf[3] = {f[5] * qK, f[6] * qK, f[7] * qK};
+      const double qt[3] = {gQ[3 * size_t(i)] * qK, gQ[3 * size_t(i) + 1] * qK, gQ[3 * size_t(i) + 2] * qK};
+      sQ.add(w, eQ[i] * qK, f[4] * qK, qt, qf);
+      ++nQ;
+    }
+  }
+  *out = rg_grid_check_t{};
+  out->n_points = n;
+  sLJ.store(&out->lennard_jones);
+  out->n_coulomb_points = nQ;
+  if (coul && nQ > 0) sQ.store(&out->coulomb);
+  return RG_OK;
+}
 }  // namespace
 
 extern "C"
@@ -833,6 +1083,13 @@ rg_status rg_grid_build_many_async(rg_ctx* ctx, const rg_forcefield* ff, const r
                                    rg_grid* const* grids)
 {
   return gridBuildMany(ctx, ff, fw, n_grids, types, probes, cutoffs, grids, false);
+}
+
+rg_status rg_grid_check(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, const rg_grid* lj_grid, int probe,
+                        const rg_grid* coulomb_grid, double probe_charge, double temperature, int n_points,
+                        const double* frac, unsigned long long seed, rg_grid_check_t* out)
+{
+  return gridCheck(ctx, ff, fw, lj_grid, probe, coulomb_grid, probe_charge, temperature, n_points, frac, seed, out);
 }
 
 rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size_t count, double* host)

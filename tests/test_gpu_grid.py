@@ -152,3 +152,53 @@ def test_interpolation_matches_oracle(gpu_ctx):
     eo, go = oracle.interpolate(fw.cell, fw.rect, dims, data, pos, gradient=True)
     assert_close(e, eo, "interpolate")
     assert np.allclose(grad, go, rtol=1e-9, atol=1e-6)
+
+
+def _check_close(g, o):
+    assert g["n_points"] == o["n_points"] and g["n_coulomb_points"] == o["n_coulomb_points"]
+    for part in ("lennard_jones", "coulomb"):
+        a, b = g[part], o[part]
+        for key in ("weight", "energy_table", "energy_full"):
+            assert abs(a[key] - b[key]) <= 1e-9 * abs(b[key]) + 1e-9, (part, key, a[key], b[key])
+        for key in ("gradient_table", "gradient_full"):
+            assert np.allclose(a[key], b[key], rtol=1e-8, atol=1e-8), (part, key, a[key], b[key])
+        assert abs(a["energy_relative_error"] - b["energy_relative_error"]) <= 1e-6 * b["energy_relative_error"]
+        assert np.allclose(a["gradient_relative_error"], b["gradient_relative_error"], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("case", ["itq29", "triclinic"])
+def test_grid_check_matches_oracle(gpu_ctx, case):
+    """rg_grid_check (the reference's test of a built grid, system_interpolation_grids.cpp:240-840) against the
+    oracle's restatement on the same grids and 2000 given points: Boltzmann weights, averages of the interpolated
+    and analytic energies and gradients, relative errors, for the LJ grid and the charge-scaled Ewald grid."""
+    print(f"case: grid check {case}", flush=True)
+    if case == "itq29":
+        spec = w.forcefield_zeolite()
+        fw3, cell = w.framework_itq29(spec, 3)
+        pos, typ, q, rect = fw3.pos[:72], fw3.type[:72], fw3.charge[:72], True
+        dims = ra.grid_dims(cell, 0.3)
+    else:
+        spec = w.forcefield_config1()
+        cell, rect = w.cell_from_lengths_angles(9.0, 10.0, 11.0, 80.0, 95.0, 105.0)
+        rng = np.random.default_rng(4)
+        pos = rng.uniform(0, 1, (10, 3)) @ cell.T
+        typ = np.array([spec.index("Si"), spec.index("O")] * 5, np.int32)
+        q = np.array([1.3, -1.3] * 5)
+        dims = [12, 13, 14]
+    probe = spec.index("C_co2")
+    fw = w.Framework(case, cell, rect, pos, typ, q)
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    lj, coul = ra.Grid(gpu_ctx, cell, rect, dims), ra.Grid(gpu_ctx, cell, rect, dims)
+    ra.build_grids(gpu_ctx, ff, f, [(lj, RG_GRID_LENNARD_JONES, probe, 12.0), (coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    frac = np.random.default_rng(21).uniform(0, 1, (2000, 3))
+    qp = float(spec.charge[probe])
+    g = ra.grid_check(gpu_ctx, ff, f, lj, probe, coul, qp, 300.0, frac=frac)
+    o = oracle.grid_check(cell, rect, dims, pos, typ, q, oracle.pairs_from_spec(spec), probe, spec.rc_framework,
+                          12.0, spec.alpha, lj.download(), frac, 300.0, coulomb_grid=coul.download(),
+                          probe_charge=qp, overlap=spec.overlap)
+    _check_close(g, o)
+    assert 0 < g["n_coulomb_points"] < g["n_points"] == 2000
+    # the default test: 1e5 Philox points, as the reference runs it after every build
+    d = ra.grid_check(gpu_ctx, ff, f, lj, probe, coul, qp, 300.0, n_points=100000, seed=5)
+    assert d["n_points"] == 100000 and np.isfinite(d["lennard_jones"]["energy_relative_error"])

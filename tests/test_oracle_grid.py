@@ -162,3 +162,64 @@ def test_tricubic_matrix_reproduces_cubic_polynomials():
     nodes = np.array([[1, 2, 3], [0, 0, 0], [1, 1, 1]]) / nc
     en = oracle.interpolate(cell, True, dims, grid, nodes @ cell.T)
     assert np.allclose(en, [poly(*s) for s in nodes], atol=1e-12)
+
+
+def _checked_grids(spacing_dims, triclinic=True, alpha=0.3):
+    cell, rect, fpos, ftype, fq, pairs = small_system(triclinic)
+    rc, probe = 6.0, 2
+    lj = oracle.grid_build(cell, rect, spacing_dims, 0, fpos, ftype, fq, pairs, probe, rc, rc, alpha)
+    q = oracle.grid_build(cell, rect, spacing_dims, 3, fpos, ftype, fq, pairs, 0, rc, rc, alpha)
+    return cell, rect, fpos, ftype, fq, pairs, rc, alpha, probe, lj, q
+
+
+def test_grid_check_is_exact_at_nodes():
+    """The grid test (system_interpolation_grids.cpp:240-840) at grid nodes: the tricubic interpolant reproduces the
+    stored energy and fractional derivatives there, so every relative error vanishes and table equals full."""
+    dims = (9, 8, 10)
+    cell, rect, fpos, ftype, fq, pairs, rc, alpha, probe, lj, q = _checked_grids(dims)
+    rng = np.random.default_rng(5)
+    ijk = np.stack([rng.integers(0, d, 200) for d in dims], 1)
+    frac = ijk / (np.array(dims) - 1.0)
+    r = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, probe, rc, rc, alpha, lj, frac, 300.0,
+                          coulomb_grid=q, probe_charge=-0.4)
+    for part in ("lennard_jones", "coulomb"):
+        s = r[part]
+        assert s["energy_relative_error"] < 1e-12, (part, s)
+        assert np.all(s["gradient_relative_error"] < 1e-10), (part, s)
+        assert abs(s["energy_table"] - s["energy_full"]) <= 1e-10 * abs(s["energy_full"])
+    assert 0 < r["n_coulomb_points"] <= r["n_points"] == 200
+
+
+def test_grid_check_weights_follow_energy_at_position():
+    """Weights: exp(-beta E) with beta = 1/(k_B T) in inverse internal units and E the analytic LJ energy in K,
+    E from calculateEnergyAtPosition (an independent per-point restatement); the Coulomb statistics count the
+    points whose E (K) is below the overlap criterion."""
+    dims = (9, 8, 10)
+    cell, rect, fpos, ftype, fq, pairs, rc, alpha, probe, lj, q = _checked_grids(dims)
+    frac = np.random.default_rng(9).uniform(0, 1, (150, 3))
+    overlap = 3000.0
+    r = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, probe, rc, rc, alpha, lj, frac, 250.0,
+                          coulomb_grid=q, probe_charge=0.7, overlap=overlap)
+    ek = np.array([oracle.energy_at_position(cell, rect, 0, fpos, ftype, fq, pairs, probe, rc, rc, alpha, cell @ s)
+                   for s in frac]) * U["energy_to_kelvin"]
+    beta = 1.0 / (U["kB"] * 250.0)
+    w = np.exp(-beta * ek)
+    assert abs(r["lennard_jones"]["weight"] - w.sum()) <= 1e-12 * w.sum()
+    assert abs(r["lennard_jones"]["energy_full"] - (w * ek).sum() / w.sum()) <= 1e-9 * abs(r["lennard_jones"]["energy_full"])
+    assert r["n_coulomb_points"] == int((ek < overlap).sum())
+
+
+def test_grid_check_error_falls_as_h4():
+    """At random points the tricubic interpolant converges to the analytic energy as h^4: halving the spacing
+    divides the relative error by about 16 (at least 8 here, with the Boltzmann weighting).  alpha rc = 3.6 makes
+    the jumps of the truncated real-space sum at the cutoff (erfc(3.6) ~ 4e-7) negligible; at alpha rc = 1.8 they
+    dominate the Ewald error and it no longer converges."""
+    frac = np.random.default_rng(13).uniform(0, 1, (300, 3))
+    errs = []
+    for dims in ((13, 13, 13), (25, 25, 25)):
+        cell, rect, fpos, ftype, fq, pairs, rc, alpha, probe, lj, q = _checked_grids(dims, False, 0.6)
+        r = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, probe, rc, rc, alpha, lj, frac, 300.0,
+                              coulomb_grid=q, probe_charge=1.0)
+        errs.append((r["lennard_jones"]["energy_relative_error"], r["coulomb"]["energy_relative_error"]))
+    for coarse, fine in zip(*errs):
+        assert fine < coarse / 8.0, errs
