@@ -303,6 +303,10 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
 /* Runs 'cycles' MC cycles; sample != 0 accumulates block averages; use_graph != 0 replays one captured cycle.
  * On return the systems' adsorbate arrays, structure factors, adsorbate count and net charge are the final state. */
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);
+/* Launch shape of rg_mc_run (no counterpart in the reference): -1 (default) lets the autotuner choose; 0, 1, 2 run
+ * one wavefront per system with 1, 2 or 4 systems per workgroup; 3 runs a team of four wavefronts per system.  Every
+ * shape computes the same trajectory bit for bit, so this only pins the speed (for reproducible timings). */
+rg_status rg_mc_set_launch_shape(rg_ctx* ctx, rg_mc* mc, int shape);
 rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
 /* Adsorbate atoms of a system (max(n_atoms) of the components slots per molecule; unused slots have type -1)
  * and the component of each molecule. */

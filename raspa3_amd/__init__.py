@@ -392,6 +392,10 @@ class Mc(_Handle):
         p = None if prob is None else L.f64(prob).reshape(-1)
         L.check(lib().rg_mc_set_proposal(self.ctx.h, self.h, system, L.i(n), L.d(p)), self.ctx.h, "set_proposal")
 
+    def set_launch_shape(self, shape):
+        """-1: autotuned; 0, 1, 2: one wavefront per system, 1, 2 or 4 systems per workgroup; 3: four per system."""
+        L.check(lib().rg_mc_set_launch_shape(self.ctx.h, self.h, int(shape)), self.ctx.h, "rg_mc_set_launch_shape")
+
     def run(self, cycles, sample=False, graph=True):
         L.check(lib().rg_mc_run(self.ctx.h, self.h, int(cycles), int(sample), int(graph)), self.ctx.h, "rg_mc_run")
 

@@ -57,6 +57,11 @@ __device__ __forceinline__ void waveSyncGlobal()
   __builtin_amdgcn_wave_barrier();
 }
 
+// Compiler-only memory barrier: values read from the per-wavefront LDS descriptors before it are read again after
+// it.  Placed at the start of each move and of the heavy helpers, it keeps the compiler from holding every
+// descriptor field in registers across the whole kernel (which spills them to scratch memory).
+__device__ __forceinline__ void reloadBarrier() { asm volatile("" ::: "memory"); }
+
 // Rotation matrix (row-major) of the unit quaternion from three uniforms (K. Shoemake, "Uniform random rotations",
 // Graphics Gems III, 1992).
 __device__ __forceinline__ void randomRotation(double u1, double u2, double u3, double Rm[9])
@@ -157,10 +162,206 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
   return e;
 }
 
+// Pair table of the force field in LDS, filled by the kernels that use the batched site energies (every system of
+// a launch shares the force field); types beyond MC_LDSTYPES read the global table.
+__shared__ rgd::Pair g_pairL[MC_LDSTYPES * MC_LDSTYPES];
+
+__device__ __forceinline__ void loadPairTable(const SysDev& S)
+{
+  if (S.ntypes <= MC_LDSTYPES)
+    for (int i = threadIdx.x; i < S.ntypes * S.ntypes; i += blockDim.x) g_pairL[i] = S.pairs[i];
+  __syncthreads();
+}
+
+// Framework atoms of a site whose type has no grid: direct sum over the whole wavefront.
+__device__ double frameworkSiteEnergy(const SysDev& S, double px, double py, double pz, int type, double q, int lane,
+                                      bool& ov)
+{
+  double e = 0.0;
+  bool o = false;
+  for (int a = lane; a < S.nfw; a += 64)
+  {
+    double dx = S.fwx[a] - px, dy = S.fwy[a] - py, dz = S.fwz[a] - pz;
+    rgd::pbc(S.box, dx, dy, dz);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < S.rcv2Fw)
+    {
+      const double v = rgd::vdw0(S.pairs[S.fwt[a] * S.ntypes + type], 1.0, rr);
+      o = o || v > S.overlap;
+      e += v;
+    }
+    if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.fwq[a] * q);
+  }
+  ov = __ballot(o) != 0ull;
+  return rgd::waveSum(e);
+}
+
+// Evaluates the candidate pairs W.pKey / W.pRR [0, fill) and adds each site's sum into W.siteT.  Keys are
+// nondecreasing in the site, so an inclusive scan that only adds across equal keys leaves every run's total in its
+// last lane; a site has at most one run per block of 64, and blocks are taken in order, so each site's sum has a
+// fixed order whatever the launch shape.
+__device__ __attribute__((always_inline)) void flushPairs(const SysDev& S, int fill, int lane, WaveScratch& W)
+{
+  waveSyncLds();
+  const bool ldsPairs = S.ntypes <= MC_LDSTYPES;
+  for (int p0 = 0; p0 < fill; p0 += 64)
+  {
+    const int p = p0 + lane;
+    const bool live = p < fill;
+    const int key = live ? W.pKey[p] : 0x7fffffff;
+    const int s = key >> 8;
+    double e = 0.0;
+    bool ov = false;
+    if (live)
+    {
+      const double rr = W.pRR[p];
+      const int l = key & 0xff;
+      const int t = W.chT[l], st = W.sType[s];
+      if (rr < S.rcv2Mol)
+      {
+        const rgd::Pair& P = ldsPairs ? g_pairL[t * S.ntypes + st] : S.pairs[t * S.ntypes + st];
+        const double v = rgd::vdw0(P, 1.0, rr);
+        ov = v > S.overlap;
+        e = v;
+      }
+      if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * W.chQ[l] * W.sQ[s]);
+    }
+    // runs are keyed by the site alone (dead lanes carry a site past every real one)
+    for (int o = 1; o < 64; o <<= 1)
+    {
+      const double t = __shfl_up(e, o, 64);
+      const int ks = __shfl_up(s, o, 64);
+      if (lane >= o && ks == s) e += t;
+    }
+    const int next = __shfl_down(s, 1, 64);
+    if (live && (lane == 63 || next != s)) W.siteT[s] += e;
+    if (live && ov) W.ovT[s] = 1;
+    waveSyncLds();
+  }
+}
+
+// External energies of ns sites (positions pos[3 s .. 3 s + 2], types W.sType, charges W.sQ) without their grid
+// part: framework atoms of types without a grid, and every adsorbate atom except those of molecules 'own' and
+// 'skip'.  Results in W.siteE[s] and W.siteOv[s] (any pair above the overlap criterion).  The adsorbate atoms are
+// read 64 at a time, once for all sites: each lane tests its atom against every site, the pairs within the cutoff
+// are compacted into LDS and evaluated densely by flushPairs.  The chunks are split into MC_PARTS partitions (one
+// per wavefront of a team of MC_PARTS, or all on one wavefront), summed separately and combined in order.
+__device__ __attribute__((always_inline)) void siteSetEnergies(const SysDev& S, int nads, int ns, const double* pos, int own, int skip, int lane,
+                                WaveScratch& W, const Team& T)
+{
+  reloadBarrier();
+  for (int s = lane; s < ns; s += 64)
+  {
+    W.siteE[s] = 0.0;
+    W.siteOv[s] = 0;
+  }
+  waveSyncLds();
+  if (S.nfw > 0)
+  {
+    for (int s = 0; s < ns; ++s)
+    {
+      const int type = W.sType[s];
+      if (S.gridOfType && S.gridOfType[type] >= 0) continue;
+      bool o;
+      const double e = frameworkSiteEnergy(S, pos[3 * s], pos[3 * s + 1], pos[3 * s + 2], type, W.sQ[s], lane, o);
+      if (lane == 0)
+      {
+        W.siteE[s] = e;
+        W.siteOv[s] = o ? 1 : 0;
+      }
+    }
+    waveSyncLds();
+  }
+  const double cut2 = S.useCharge ? fmax(S.rcv2Mol, S.rcc2) : S.rcv2Mol;
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // adsorbate chunks q, q + MC_PARTS, ... of 64 atoms into W.siteT / W.ovT
+  auto partition = [&](int q)
+  {
+    for (int s = lane; s < ns; s += 64)
+    {
+      W.siteT[s] = 0.0;
+      W.ovT[s] = 0;
+    }
+    waveSyncLds();
+    for (int a0 = 64 * q; a0 < nads; a0 += 64 * MC_PARTS)
+    {
+      const int a = a0 + lane;
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      bool valid = false;
+      if (a < nads)
+      {
+        const int t = S.at[a], m = S.amol[a];
+        valid = t >= 0 && m != own && m != skip;
+        ax = S.ax[a];
+        ay = S.ay[a];
+        az = S.az[a];
+        W.chT[lane] = t;
+        W.chQ[lane] = S.aq[a];
+      }
+      int fill = 0;
+      for (int s = 0; s < ns; ++s)
+      {
+        if (fill > MC_PAIRCAP - 64)
+        {
+          flushPairs(S, fill, lane, W);
+          fill = 0;
+        }
+        double dx = ax - pos[3 * s], dy = ay - pos[3 * s + 1], dz = az - pos[3 * s + 2];
+        rgd::pbc(S.box, dx, dy, dz);
+        const double rr = dx * dx + dy * dy + dz * dz;
+        const bool in = valid && rr < cut2;
+        const unsigned long long m = __ballot(in);
+        if (in)
+        {
+          const int idx = fill + __popcll(m & below);
+          W.pKey[idx] = (s << 8) | lane;
+          W.pRR[idx] = rr;
+        }
+        fill += __popcll(m);
+      }
+      flushPairs(S, fill, lane, W);  // before the next chunk replaces chT / chQ
+    }
+    waveSyncLds();
+  };
+  // site sums ((((framework + P0) + P1) + P2) + P3) in either team size
+  if (T.nw == 1)
+  {
+    for (int q = 0; q < MC_PARTS && 64 * q < nads; ++q)
+    {
+      partition(q);
+      for (int s = lane; s < ns; s += 64)
+      {
+        W.siteE[s] += W.siteT[s];
+        W.siteOv[s] |= W.ovT[s];
+      }
+      waveSyncLds();
+    }
+  }
+  else
+  {
+    partition(T.wi);
+    __syncthreads();
+    for (int s = lane; s < ns; s += 64)
+    {
+      double e = W.siteE[s];
+      int o = W.siteOv[s];
+      for (int q = 0; q < MC_PARTS; ++q)
+      {
+        e += T.ws[q].siteT[s];
+        o |= T.ws[q].ovT[s];
+      }
+      W.siteE[s] = e;
+      W.siteOv[s] = o;
+    }
+    __syncthreads();  // the partition arrays are read by every wavefront before any overwrites them
+  }
+}
+
 // Energies of K trial sets of n sites each (sites in LDS as x, y, z per site, set-major).  Lane j < K receives the
 // total of set j and its overlap flag.
 __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const int* atomIdx, int n, int K,
-                                 const double* pos, int own, int skip, int lane, double& Uj, bool& ovj)
+                                 const double* pos, int own, int skip, int lane, WaveScratch& W, const Team& T,
+                                 double& Uj, bool& ovj)
 {
   // grid-typed sites: lane j interpolates the sites of trial set j, so the K sets' gathers are in flight together
   double gpart = 0.0;
@@ -179,27 +380,34 @@ __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const
       if (S.useCharge && S.coulGrid >= 0) gpart += c.q[a] * rgd::interpolate(S.grids[S.coulGrid], p[0], p[1], p[2]);
     }
   }
-  // adsorbate atoms (and framework atoms of types without a grid): one site at a time over the whole wavefront
+  // adsorbate atoms (and framework atoms of types without a grid): all K * n sites in one batch
+  const int ns = K * n;
+  for (int s = 0; s < n; ++s)  // uniform loop: atomIdx stays in registers
+  {
+    const int a = atomIdx[s];
+    for (int j = lane; j < K; j += 64)
+    {
+      W.sType[j * n + s] = c.type[a];
+      W.sQ[j * n + s] = c.q[a];
+    }
+  }
+  waveSyncLds();
+  siteSetEnergies(S, nads, ns, pos, own, skip, lane, W, T);
   Uj = 0.0;
   ovj = false;
-  for (int j = 0; j < K; ++j)
+  if (lane < K)
   {
     double u = 0.0;
     bool any = false;
     for (int s = 0; s < n; ++s)
     {
-      const int a = atomIdx[s];
-      const double* p = pos + 3 * (j * n + s);
-      bool o;
-      u += siteExternal(S, nads, p[0], p[1], p[2], c.type[a], c.q[a], own, skip, lane, o, false);
-      any = any || o;
+      u += W.siteE[lane * n + s];
+      any = any || W.siteOv[lane * n + s] != 0;
     }
-    if (lane == j)
-    {
-      Uj = u + gpart;
-      ovj = any || gov;
-    }
+    Uj = u + gpart;
+    ovj = any || gov;
   }
+  waveSyncLds();
 }
 
 // Rosenbluth weight over the lanes (lane j holds trial j): logW = log(sum_j w_j / K) with w_j = exp(-beta U_j) * bias_j
@@ -242,8 +450,14 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 }
 
 // atoms: x, y, z, q (q carries the sign: + new, - old); tab: per atom 3 x (EW_MAXK + 1) complex.
-__device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, double2* tab, int lane, bool writeTrial)
+// Passes of 256 k-vectors (four per lane: their loads are issued together, so one memory latency covers four);
+// pass p belongs to partition p % MC_PARTS, and a wavefront of a team of MC_PARTS runs its own partition only.  The
+// trial structure factor of a pass is written, and later committed by waveAcceptEwald, by the same wavefront.
+__device__ __attribute__((always_inline)) double waveEwaldDelta(const SysDev& S, int na, const double* at, double2* tab,
+                                                                int lane, bool writeTrial, WaveScratch& W,
+                                                                const Team& T)
 {
+  reloadBarrier();
   const int stride = 3 * (EW_MAXK + 1);
   for (int t = lane; t < na * 3; t += 64)
   {
@@ -263,67 +477,86 @@ __device__ double waveEwaldDelta(const SysDev& S, int na, const double* at, doub
     }
   }
   waveSyncLds();
-  // four k-vectors per lane and pass: their loads are issued together, so one memory latency covers four
   constexpr int U = 4;
-  double de = 0.0;
-  for (int k0 = 0; k0 < S.nk; k0 += 64 * U)
+  const int nk = S.nk;
+  const int npass = (nk + 64 * U - 1) / (64 * U);
+  auto partition = [&](int q)
   {
-    int code[U];
-    double2 sk[U];
-    double fk[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
+    double de = 0.0;
+    for (int pass = q; pass < npass; pass += MC_PARTS)
     {
-      const int k = k0 + u * 64 + lane;
-      const bool in = k < S.nk;
-      code[u] = in ? S.kcode[k] : (128 << 8) | (128 << 16);  // padding decodes to k = (0, 0, 0)
-      sk[u] = in ? S.Sk[k] : make_double2(0.0, 0.0);
-      fk[u] = in ? S.fk[k] : 0.0;
-    }
+      const int k0 = pass * 64 * U;
+      int code[U];
+      double2 sk[U];
+      double fk[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-    {
-      const int k = k0 + u * 64 + lane;
-      const int kx = code[u] & 0xff, ky = ((code[u] >> 8) & 0xff) - 128, kz = ((code[u] >> 16) & 0xff) - 128;
-      double2 d = make_double2(0.0, 0.0);
-      for (int a = 0; a < na; ++a)
+      for (int u = 0; u < U; ++u)
       {
-        const double2* row = tab + a * stride;
-        const double2 ex = row[kx];
-        double2 ey = row[(EW_MAXK + 1) + (ky < 0 ? -ky : ky)];
-        if (ky < 0) ey.y = -ey.y;
-        double2 ez = row[2 * (EW_MAXK + 1) + (kz < 0 ? -kz : kz)];
-        if (kz < 0) ez.y = -ez.y;
-        const double2 e = cmul(cmul(ex, ey), ez);
-        d.x += at[4 * a + 3] * e.x;
-        d.y += at[4 * a + 3] * e.y;
+        const int k = k0 + u * 64 + lane;
+        const bool in = k < nk;
+        code[u] = in ? S.kcode[k] : (128 << 8) | (128 << 16);  // padding decodes to k = (0, 0, 0)
+        sk[u] = in ? S.Sk[k] : make_double2(0.0, 0.0);
+        fk[u] = in ? S.fk[k] : 0.0;
       }
-      if (k >= S.nk) continue;
-      const double2 t = make_double2(sk[u].x + d.x, sk[u].y + d.y);
-      de += fk[u] * ((t.x * t.x + t.y * t.y) - (sk[u].x * sk[u].x + sk[u].y * sk[u].y));
-      if (writeTrial) S.St[k] = t;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+      {
+        const int k = k0 + u * 64 + lane;
+        const int kx = code[u] & 0xff, ky = ((code[u] >> 8) & 0xff) - 128, kz = ((code[u] >> 16) & 0xff) - 128;
+        double2 d = make_double2(0.0, 0.0);
+        for (int a = 0; a < na; ++a)
+        {
+          const double2* row = tab + a * stride;
+          const double2 ex = row[kx];
+          double2 ey = row[(EW_MAXK + 1) + (ky < 0 ? -ky : ky)];
+          if (ky < 0) ey.y = -ey.y;
+          double2 ez = row[2 * (EW_MAXK + 1) + (kz < 0 ? -kz : kz)];
+          if (kz < 0) ez.y = -ez.y;
+          const double2 e = cmul(cmul(ex, ey), ez);
+          d.x += at[4 * a + 3] * e.x;
+          d.y += at[4 * a + 3] * e.y;
+        }
+        if (k >= nk) continue;
+        const double2 t = make_double2(sk[u].x + d.x, sk[u].y + d.y);
+        de += fk[u] * ((t.x * t.x + t.y * t.y) - (sk[u].x * sk[u].x + sk[u].y * sk[u].y));
+        if (writeTrial) S.St[k] = t;
+      }
     }
-  }
-  return rgd::waveSum(de);
-}
-__device__ void waveAcceptEwald(const SysDev& S, int lane)
-{
-  constexpr int U = 4;
-  for (int k0 = 0; k0 < S.nk; k0 += 64 * U)
+    return rgd::waveSum(de);
+  };
+  // ((P0 + P1) + P2) + P3 in either team size
+  if (T.nw == 1)
   {
-    double2 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-    {
-      const int k = k0 + u * 64 + lane;
-      if (k < S.nk) v[u] = S.St[k];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u)
-    {
-      const int k = k0 + u * 64 + lane;
-      if (k < S.nk) S.Sk[k] = v[u];
-    }
+    double tot = 0.0;
+    for (int q = 0; q < MC_PARTS; ++q) tot += partition(q);
+    return tot;
+  }
+  const double mine = partition(T.wi);
+  if (lane == 0) W.deT = mine;
+  __syncthreads();
+  double tot = 0.0;
+  for (int q = 0; q < MC_PARTS; ++q) tot += T.ws[q].deT;
+  __syncthreads();
+  return tot;
+}
+
+// Commits the trial structure factor: each wavefront copies the passes it computed.
+__device__ void waveAcceptEwald(const SysDev& S, int lane, const Team& T)
+{
+  // four k-vectors per lane and pass, loaded before any is stored (indices clamped, so no load is conditional)
+  const int nk = S.nk;
+  const double2* __restrict__ src = S.St;
+  double2* __restrict__ dst = S.Sk;
+  const int npass = (nk + 255) / 256;
+  for (int pass = (T.nw == 1 ? 0 : T.wi); pass < npass; pass += (T.nw == 1 ? 1 : MC_PARTS))
+  {
+    const int k = pass * 256 + lane;
+    const double2 v0 = src[min(k, nk - 1)], v1 = src[min(k + 64, nk - 1)];
+    const double2 v2 = src[min(k + 128, nk - 1)], v3 = src[min(k + 192, nk - 1)];
+    if (k < nk) dst[k] = v0;
+    if (k + 64 < nk) dst[k + 64] = v1;
+    if (k + 128 < nk) dst[k + 128] = v2;
+    if (k + 192 < nk) dst[k + 192] = v3;
   }
 }
 
@@ -377,8 +610,8 @@ __device__ double proposalLogBias(const McSys& Y, const SysDev& S, const double 
 // retrace), Rosenbluth selection.  On success every lane holds the selected position, its energy and log weight,
 // and the second selection uniform of the move.
 __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                              unsigned long long step, int retrace, int lane, WaveScratch& W, double& fx, double& fy,
-                              double& fz, double& Ufirst, double& logW1, double& uSel2)
+                              unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& fx,
+                              double& fy, double& fz, double& Ufirst, double& logW1, double& uSel2)
 {
   const int K1 = R.kFirst;
   const int own = retrace >= 0 ? retrace : -2;
@@ -405,7 +638,7 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
   double U1;
   bool ov1;
   const int fbIdx[1] = {fb};
-  trialSetEnergies(S, nads, c, fbIdx, 1, K1, W.pos, own, -2, lane, U1, ov1);
+  trialSetEnergies(S, nads, c, fbIdx, 1, K1, W.pos, own, -2, lane, W, T, U1, ov1);
   double uSel1;
   uni(R, sys, step, SLOT_SELECT, uSel1, uSel2);
   int sel1;
@@ -425,13 +658,14 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
 // weight and the external energy of the grown molecule.  'retrace' >= 0 retraces that existing molecule instead
 // (trial 0 = its current first bead and orientation; its own atoms are excluded).
 __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+                          unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& logW,
+                          double& Uext)
 {
   const int K2 = R.kOrient;
   const int own = retrace >= 0 ? retrace : -2;
   const int fb = c.first;
   double fx, fy, fz, Ufirst, logW1, uSel2;
-  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, fx, fy, fz, Ufirst, logW1, uSel2)) return false;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, T, fx, fy, fz, Ufirst, logW1, uSel2)) return false;
   if (c.na == 1)
   {
     if (lane == 0)
@@ -480,7 +714,7 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
   waveSyncLds();
   double U2;
   bool ov2;
-  trialSetEnergies(S, nads, c, idx, n, K2, W.pos, own, -2, lane, U2, ov2);
+  trialSetEnergies(S, nads, c, idx, n, K2, W.pos, own, -2, lane, W, T, U2, ov2);
   int sel2;
   double logW2;
   waveRosenbluth(U2, ov2, 0.0, K2, R.beta, uSel2, lane, sel2, logW2);
@@ -623,7 +857,7 @@ __device__ void chainTrial(const McRun& R, const Comp& c, int sys, unsigned long
 }
 
 // bend and torsion energy of bead i against beads i-3 .. i-1 (mol, 3 per bead)
-__device__ double chainBonded(const Comp& c, const double* mol, int i)
+__device__ __attribute__((always_inline)) double chainBonded(const Comp& c, const double* mol, int i)
 {
   double e = 0.0;
   if (i >= 2)
@@ -680,12 +914,13 @@ __device__ double chainIntra(const SysDev& S, const Comp& c, const double* mol, 
 // molecule, then kOrient trials for each of beads 1 .. na-1.  Writes the chain into W.mol; Uext is the external
 // plus intra-molecular (non-bonded and bonded) energy.
 __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                          unsigned long long step, int retrace, int lane, WaveScratch& W, double& logW, double& Uext)
+                          unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& logW,
+                          double& Uext)
 {
   const int K = R.kOrient;
   const int own = retrace >= 0 ? retrace : -2;
   double fx, fy, fz, Ufirst, lw, uSel2;
-  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, fx, fy, fz, Ufirst, lw, uSel2)) return false;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, T, fx, fy, fz, Ufirst, lw, uSel2)) return false;
   if (lane == 0)
   {
     W.mol[0] = fx;
@@ -717,7 +952,7 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
     double Ui;
     bool ovi;
     const int bi[1] = {i};
-    trialSetEnergies(S, nads, c, bi, 1, K, W.pos, own, -2, lane, Ui, ovi);
+    trialSetEnergies(S, nads, c, bi, 1, K, W.pos, own, -2, lane, W, T, Ui, ovi);
     if (lane < K) Ui += chainIntra(S, c, W.mol, i, p, ovi);
     double us, unused;
     uni(R, sys, step, SLOT_CHAIN - 16 + i, us, unused);
@@ -747,17 +982,17 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
 template <bool CHAIN>
 __device__ __forceinline__ bool growMolecule(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys,
                                              int nads, unsigned long long step, int retrace, int lane, WaveScratch& W,
-                                             double& logW, double& Uext)
+                                             const Team& T, double& logW, double& Uext)
 {
-  if (CHAIN && c.flexible) return growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
-  return growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, logW, Uext);
+  if (CHAIN && c.flexible) return growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, T, logW, Uext);
+  return growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, T, logW, Uext);
 }
 
 
 // Ewald energy change of inserting (+1) or removing (-1) the atoms in W.mol of component c, incl. the constant
 // self and intra-molecular exclusion terms of the rigid body.
-__device__ double ewaldInsertRemove(const McRun& R, const SysDev& S, double netCharge, const Comp& c, int sign, int lane,
-                                    WaveScratch& W, bool writeTrial)
+__device__ __attribute__((always_inline)) double ewaldInsertRemove(const McRun& R, const SysDev& S, double netCharge, const Comp& c, int sign, int lane,
+                                    WaveScratch& W, const Team& T, bool writeTrial)
 {
   if (!S.useCharge || !c.charged) return 0.0;
   if (lane < c.na)
@@ -768,7 +1003,7 @@ __device__ double ewaldInsertRemove(const McRun& R, const SysDev& S, double netC
     W.at[4 * lane + 3] = sign * c.q[lane];
   }
   waveSyncLds();
-  const double df = waveEwaldDelta(S, c.na, W.at, W.tab, lane, writeTrial);
+  const double df = waveEwaldDelta(S, c.na, W.at, W.tab, lane, writeTrial, W, T);
   // net charge: neutral molecules leave it unchanged; otherwise (Q + dq)^2 - Q^2 times the single-ion term
   const double dq = sign * c.qsum;
   const double uIon = -(S.singleIon - S.coulomb * S.alpha / sqrt(M_PI));
@@ -786,15 +1021,19 @@ __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, con
 {
   __shared__ WaveScratch scratch[4];
   const McRun& R = *Rp;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront index is uniform over the wavefront; readfirstlane tells the compiler so, and the per-system
+  // structures are then read through the scalar cache instead of per-lane vector loads
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + w;
+  loadPairTable(*Sp);
   if (i >= count) return;
   const SysDev& S = *Sp;
   const McSys& Y = *Yp;
   const Comp& c = R.comp[comp];
   double logW = -INFINITY, U = 0.0, de = 0.0;
-  if (growMolecule<CHAIN>(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], logW, U))
-    de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], false);
+  const Team T{1, 0, &scratch[w]};
+  if (growMolecule<CHAIN>(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], T, logW, U))
+    de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], T, false);
   else
   {
     // a failed growth (every trial of a step overlapping) has weight zero, whatever partial sums it left
@@ -872,8 +1111,10 @@ __global__ __launch_bounds__(256) void k_molecule_weights(const SysDev* __restri
 }
 
 template <bool CHAIN>
-__device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W)
+__device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W,
+                         const Team& T)
 {
+  reloadBarrier();
   const unsigned long long step = (unsigned long long)L.step;
   double uMove, uComp, uMol, uAcc;
   uni(R, sys, step, SLOT_MOVE, uMove, uComp);
@@ -958,14 +1199,20 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     double Uold = rgd::waveSum(lane < cm.na ? gl : 0.0);
     double Unew = rgd::waveSum(newLane ? gl : 0.0);
     bool ovNew = __ballot(newLane && gov) != 0ull;
+    // the other molecules (and framework atoms of types without a grid): the 2 na sites in one batch
+    if (lane < 2 * cm.na)
+    {
+      const int a = lane < cm.na ? lane : lane - cm.na;
+      W.sType[lane] = cm.type[a];
+      W.sQ[lane] = cm.q[a];
+    }
+    waveSyncLds();
+    siteSetEnergies(S, L.nads, 2 * cm.na, W.pos, m, -2, lane, W, T);
     for (int a = 0; a < cm.na; ++a)
     {
-      bool o;
-      Uold += siteExternal(S, L.nads, W.pos[3 * a], W.pos[3 * a + 1], W.pos[3 * a + 2], cm.type[a], cm.q[a], m, -2,
-                           lane, o, false);
-      Unew += siteExternal(S, L.nads, W.pos[3 * (cm.na + a)], W.pos[3 * (cm.na + a) + 1], W.pos[3 * (cm.na + a) + 2],
-                           cm.type[a], cm.q[a], m, -2, lane, o, false);
-      ovNew = ovNew || o;
+      Uold += W.siteE[a];
+      Unew += W.siteE[cm.na + a];
+      ovNew = ovNew || W.siteOv[cm.na + a] != 0;
     }
     if (ovNew) return;
     double de = 0.0;
@@ -982,19 +1229,19 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
         W.at[4 * lane + 3] = isNew ? cm.q[a] : -cm.q[a];
       }
       waveSyncLds();
-      de = waveEwaldDelta(S, 2 * cm.na, W.at, W.tab, lane, true);
+      de = waveEwaldDelta(S, 2 * cm.na, W.at, W.tab, lane, true, W, T);
     }
     const double dU = Unew - Uold + de;
     if (uAcc < exp(-R.beta * dU))
     {
-      if (lane < cm.na)
+      if (T.wi == 0 && lane < cm.na)  // adsorbate arrays: written by the team's first wavefront
       {
         const int a0 = m * R.slots + lane;
         S.ax[a0] = W.pos[3 * (cm.na + lane)];
         S.ay[a0] = W.pos[3 * (cm.na + lane) + 1];
         S.az[a0] = W.pos[3 * (cm.na + lane) + 2];
       }
-      if (ew) waveAcceptEwald(S, lane);
+      if (ew) waveAcceptEwald(S, lane, T);
       L.U += dU;
       L.accepted[isTrans ? 2 : 3] += 1;
     }
@@ -1033,8 +1280,8 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     if (m < 0) return;
   }
   double logW, Uext;
-  if (!growMolecule<CHAIN>(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, logW, Uext)) return;
-  const double de = ewaldInsertRemove(R, S, L.netCharge, c, insert ? +1 : -1, lane, W, true);
+  if (!growMolecule<CHAIN>(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, T, logW, Uext)) return;
+  const double de = ewaldInsertRemove(R, S, L.netCharge, c, insert ? +1 : -1, lane, W, T, true);
   if (insert)
   {
     // Pacc = exp(-beta dE_Ewald) beta f V / (N + 1) W / W_IG (insertion_cbmc.cpp:157-168); W_IG = 1 for rigid bodies
@@ -1042,7 +1289,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     if (log(uAcc) < logPacc)
     {
       const int mn = L.nmol;
-      if (lane < R.slots)
+      if (T.wi == 0 && lane < R.slots)
       {
         const int a0 = mn * R.slots + lane;
         const bool real = lane < c.na;
@@ -1053,8 +1300,8 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
         S.at[a0] = real ? c.type[lane] : -1;
         S.amol[a0] = mn;
       }
-      if (lane == 0) Y.comp[mn] = ci;
-      if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
+      if (T.wi == 0 && lane == 0) Y.comp[mn] = ci;
+      if (S.useCharge && c.charged) waveAcceptEwald(S, lane, T);
       L.nmol = mn + 1;
       L.ncomp[ci] = nc + 1;
       L.nads = (mn + 1) * R.slots;
@@ -1068,12 +1315,12 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
   const double logPacc = -R.beta * de + log(double(nc) / fugVol) - logW + c.logWIG;
   if (log(uAcc) < logPacc)
   {
-    if (S.useCharge && c.charged) waveAcceptEwald(S, lane);
+    if (S.useCharge && c.charged) waveAcceptEwald(S, lane, T);
     // move the last molecule into slot m
     const int last = L.nmol - 1;
     if (m != last)
     {
-      if (lane < R.slots)
+      if (T.wi == 0 && lane < R.slots)
       {
         const int d = m * R.slots + lane, s = last * R.slots + lane;
         S.ax[d] = S.ax[s];
@@ -1082,7 +1329,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
         S.aq[d] = S.aq[s];
         S.at[d] = S.at[s];
       }
-      if (lane == 0) Y.comp[m] = Y.comp[last];
+      if (T.wi == 0 && lane == 0) Y.comp[m] = Y.comp[last];
     }
     L.nmol = last;
     L.ncomp[ci] = nc - 1;
@@ -1097,15 +1344,34 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
 // spreads them over all compute units.  Dynamic LDS: one WaveScratch per wavefront.
 template <bool CHAIN>
 __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
-                                              McSys* __restrict__ Yarr, int nsys, int cycles, int sample)
+                                              McSys* __restrict__ Yarr, int nsys, int cycles, int sample, int nw)
 {
   extern __shared__ WaveScratch scratch[];
+  // each wavefront's system descriptors, copied into LDS: the moves read their fields constantly, and LDS reads
+  // stay cheap across the fences that order the wavefront's LDS traffic (global reads would go back to memory)
+  __shared__ SysDev sdev[4];
+  __shared__ McSys ysys[4];
   const McRun& R = *Rp;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int sys = blockIdx.x * (blockDim.x >> 6) + w;
-  if (sys >= nsys) return;
-  SysDev& S = Sarr[sys];
-  McSys& Y = Yarr[sys];
+  // the wavefront index is uniform over the wavefront; readfirstlane tells the compiler so, and the per-system
+  // structures are then read through the scalar cache instead of per-lane vector loads
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  // nw wavefronts (1 or MC_PARTS) per system, blockDim.x / (64 nw) systems per workgroup
+  const int slot = w / nw, wi = w % nw;
+  const int sys = blockIdx.x * (blockDim.x / (64 * nw)) + slot;
+  loadPairTable(Sarr[0]);  // the systems of a launch share the force field
+  if (sys >= nsys) return;  // a whole team returns: its wavefronts share sys
+  const Team T{nw, wi, &scratch[slot * nw]};
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&Sarr[sys]);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&sdev[w]);
+    for (int i = lane; i < int(sizeof(SysDev) / 4); i += 64) dst[i] = src[i];
+    const uint32_t* ysrc = reinterpret_cast<const uint32_t*>(&Yarr[sys]);
+    uint32_t* ydst = reinterpret_cast<uint32_t*>(&ysys[w]);
+    for (int i = lane; i < int(sizeof(McSys) / 4); i += 64) ydst[i] = ysrc[i];
+    waveSyncLds();
+  }
+  const SysDev& S = sdev[w];
+  const McSys& Y = ysys[w];
   Live L;
   L.nmol = Y.nmol;
   for (int q = 0; q < MC_MAXCOMP; ++q) L.ncomp[q] = Y.ncomp[q];
@@ -1125,17 +1391,21 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
     const int nsteps = max(R.minStepsPerCycle, L.nmol);
     for (int s = 0; s < nsteps; ++s)
     {
-      gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w]);
-      waveSyncGlobal();
+      gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w], T);
+      // the team's first wavefront wrote the adsorbate arrays; the next step reads them on every wavefront
+      if (nw > 1)
+        __syncthreads();
+      else
+        waveSyncGlobal();
       L.step += 1;
     }
     if (sample)
     {
-      // block sums live in global memory, read and written by lane 0 only
-      if (lane == 0)
+      // block sums live in global memory, read and written by lane 0 of the team's first wavefront only
+      if (wi == 0 && lane == 0)
       {
         const int b = min(MC_BLOCKS - 1, int(L.cycle * MC_BLOCKS / max(1LL, R.productionCycles)));
-        McBlock& B = Y.block[b];
+        McBlock& B = Yarr[sys].block[b];
         B.count += 1.0;
         B.U += L.U;
         for (int q = 0; q < R.ncomp; ++q)
@@ -1149,19 +1419,21 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
       L.cycle += 1;
     }
   }
-  if (lane == 0)
+  if (wi == 0 && lane == 0)
   {
-    Y.nmol = L.nmol;
-    for (int q = 0; q < MC_MAXCOMP; ++q) Y.ncomp[q] = L.ncomp[q];
-    S.nads = L.nads;
-    Y.U = L.U;
-    S.netCharge = L.netCharge;
-    Y.step = L.step;
-    Y.cycle = L.cycle;
+    McSys& Yg = Yarr[sys];
+    SysDev& Sg = Sarr[sys];
+    Yg.nmol = L.nmol;
+    for (int q = 0; q < MC_MAXCOMP; ++q) Yg.ncomp[q] = L.ncomp[q];
+    Sg.nads = L.nads;
+    Yg.U = L.U;
+    Sg.netCharge = L.netCharge;
+    Yg.step = L.step;
+    Yg.cycle = L.cycle;
     for (int q = 0; q < 4; ++q)
     {
-      Y.stat.trials[q] = L.trials[q];
-      Y.stat.accepted[q] = L.accepted[q];
+      Yg.stat.trials[q] = L.trials[q];
+      Yg.stat.accepted[q] = L.accepted[q];
     }
   }
 }
@@ -1173,7 +1445,9 @@ __global__ __launch_bounds__(256) void k_total_energy(const McRun* __restrict__ 
                                                       const McSys* __restrict__ Yarr, int nsys, double* out)
 {
   const McRun& R = *Rp;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wavefront index is uniform over the wavefront; readfirstlane tells the compiler so, and the per-system
+  // structures are then read through the scalar cache instead of per-lane vector loads
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int sys = blockIdx.x * 4 + w;
   if (sys >= nsys) return;
   const SysDev& S = Sarr[sys];
@@ -1246,7 +1520,7 @@ template __global__ void k_widom<false>(const McRun*, const SysDev*, const McSys
                                         double*, double*);
 template __global__ void k_widom<true>(const McRun*, const SysDev*, const McSys*, int, long long, int, double*,
                                        double*, double*);
-template __global__ void k_gcmc<false>(const McRun*, SysDev*, McSys*, int, int, int);
-template __global__ void k_gcmc<true>(const McRun*, SysDev*, McSys*, int, int, int);
+template __global__ void k_gcmc<false>(const McRun*, SysDev*, McSys*, int, int, int, int);
+template __global__ void k_gcmc<true>(const McRun*, SysDev*, McSys*, int, int, int, int);
 
 }  // namespace rg

@@ -95,12 +95,46 @@ struct McRun
 };
 
 // LDS of one wavefront in the Monte Carlo kernels: trial atom positions, Ewald eik tables and the grown molecule.
+// sites of one energy batch (K trial sets of up to MC_MAXRIGID - 1 sites), candidate pairs per flush, and force
+// fields whose pair table is copied into LDS (larger ones read it from global memory).  The scratch is kept under
+// 20 KB per wavefront so that two workgroups of four wavefronts fit the 160 KB of LDS of a compute unit.
+#define MC_MAXSITES (64 * (MC_MAXRIGID - 1))
+#define MC_PAIRCAP 128
+#define MC_LDSTYPES 12
+
 struct WaveScratch
 {
   double pos[64 * 3 * (MC_MAXRIGID - 1)];
   double at[4 * 2 * MC_MAXRIGID];
   double2 tab[2 * MC_MAXRIGID * 3 * (EW_MAXK + 1)];
   double mol[3 * MC_MAXATOMS];
+  // batched site energies: per-site charge, energy and the current partition's sum (siteT); candidate pairs
+  // (r^2, key = site << 8 | lane) of the current chunk of 64 adsorbate atoms, whose charges and types are chQ / chT
+  double siteE[MC_MAXSITES];
+  double sQ[MC_MAXSITES];
+  double siteT[MC_MAXSITES];
+  double pRR[MC_PAIRCAP];
+  double chQ[64];
+  double deT;  // one partition's Ewald difference
+  short sType[MC_MAXSITES];
+  unsigned short pKey[MC_PAIRCAP];
+  unsigned char siteOv[MC_MAXSITES];
+  unsigned char ovT[MC_MAXSITES];
+  unsigned char chT[64];
+};
+static_assert(sizeof(WaveScratch) <= 19 * 1024, "four WaveScratch and the LDS pair table must fit in 80 KB");
+
+// The heavy loops of a move (adsorbate atom chunks, Ewald k-vector passes) are cut into MC_PARTS fixed partitions
+// whose results are combined in partition order, so a system's trajectory is the same bit for bit whether one
+// wavefront runs all partitions or a team of MC_PARTS wavefronts runs one each.
+#define MC_PARTS 4
+
+// The wavefronts that run one system: nw (1 or MC_PARTS) of them, this one is wi; ws is the team's first
+// WaveScratch (the team's are consecutive).
+struct Team
+{
+  int nw, wi;
+  WaveScratch* ws;
 };
 
 }  // namespace rg

@@ -10,7 +10,7 @@
 namespace rg
 {
 template <bool CHAIN>
-__global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample);
+__global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample, int nw);
 template <bool CHAIN>
 __global__ void k_widom(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first, int count,
                         double* outLogW, double* outU, double* outEwald);
@@ -35,7 +35,8 @@ struct rg_mc
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphSample = -1;
-  int graphWpb = 0;
+  int graphShape = -1;  // launch-shape candidate the graph was captured with
+  int forcedShape = -1;  // rg_mc_set_launch_shape: fixed candidate, or -1 for the autotuner
   int computeUnits = 256;
 };
 
@@ -46,6 +47,8 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
                    const rg_mc_params* p, McRun& R)
 {
   if (ncomp < 1 || ncomp > MC_MAXCOMP) return fail(ctx, RG_ERR_UNSUPPORTED, "1..%d components", MC_MAXCOMP);
+  // adsorbate atom types are staged in LDS as bytes
+  if (ff->n > 255) return fail(ctx, RG_ERR_UNSUPPORTED, "the Monte Carlo engine takes at most 255 pseudo-atom types");
   const Units& u = units();
   std::memset(&R, 0, sizeof(R));
   R.seed = p->seed;
@@ -284,56 +287,66 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
   return RG_OK;
 }
 
+rg_status rg_mc_set_launch_shape(rg_ctx* ctx, rg_mc* mc, int shape)
+{
+  if (!ctx || !mc || shape < -1 || shape > 3) return fail(ctx, RG_ERR_INVALID, "rg_mc_set_launch_shape: shape in [-1, 3]");
+  mc->forcedShape = shape;
+  return RG_OK;
+}
+
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph)
 {
   if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
   RG_ENTER(ctx);
   const int nsys = static_cast<int>(mc->systems.size());
-  // Systems per workgroup (1, 2 or 4 wavefronts) is tuned online: while the key is open, cycles of the real run
-  // are launched eagerly with each candidate and timed.  Every candidate computes the same trajectory (one
-  // wavefront per system in every shape), so the choice only affects speed.
-  static const int kWpb[3] = {1, 2, 4};
+  // The launch shape is tuned online: one wavefront per system with 1, 2 or 4 systems per workgroup, or a team of
+  // MC_PARTS wavefronts per system (one system per workgroup).  While the key is open, cycles of the real run are
+  // launched eagerly with each candidate and timed.  Every candidate computes the same trajectory bit for bit (the
+  // heavy loops are cut into MC_PARTS fixed partitions combined in a fixed order), so the choice only affects speed.
+  static const int kNw[4] = {1, 1, 1, MC_PARTS};
+  static const int kWpb[4] = {1, 2, 4, 1};
   bool chain = false;
   for (int c = 0; c < mc->run.ncomp; ++c) chain = chain || mc->run.comp[c].flexible;
-  auto launch = [&](int wpb)
+  auto launch = [&](int cand)
   {
-    hipLaunchKernelGGL(chain ? k_gcmc<true> : k_gcmc<false>, dim3((nsys + wpb - 1) / wpb), dim3(64 * wpb),
-                       wpb * sizeof(WaveScratch), ctx->stream,
-                       mc->dRun, mc->dS, mc->dY, nsys, 1, sample);
+    const int nw = kNw[cand], wpb = kWpb[cand];
+    hipLaunchKernelGGL(chain ? k_gcmc<true> : k_gcmc<false>, dim3((nsys + wpb - 1) / wpb), dim3(64 * nw * wpb),
+                       nw * wpb * sizeof(WaveScratch), ctx->stream,
+                       mc->dRun, mc->dS, mc->dY, nsys, 1, sample, nw);
     return hipGetLastError();
   };
   Autotuner& tuner = Autotuner::instance();
   const std::string key = "k_gcmc/systems=" + std::to_string(nsys) + "/cu=" + std::to_string(mc->computeUnits);
   int done = 0;
-  for (int cand; done < cycles && (cand = tuner.next(key, 3, 2)) >= 0; ++done)
+  for (int cand; mc->forcedShape < 0 && done < cycles && (cand = tuner.next(key, 4, 2)) >= 0; ++done)
   {
     float ms = 0.0f;
-    RG_HIP(ctx, timeLaunch(ctx->stream, [&] { return launch(kWpb[cand]); }, ms));
+    RG_HIP(ctx, timeLaunch(ctx->stream, [&] { return launch(cand); }, ms));
     tuner.report(key, cand, ms);
   }
   const int best = tuner.best(key);
-  const int wpb = kWpb[best >= 0 ? best : 0];
+  const int shape = mc->forcedShape >= 0 ? mc->forcedShape : (best >= 0 ? best : 0);
   if (!use_graph)
   {
-    for (int c = done; c < cycles; ++c) RG_HIP(ctx, launch(wpb));
+    for (int c = done; c < cycles; ++c) RG_HIP(ctx, launch(shape));
   }
   else if (done < cycles)
   {
     // one MC cycle of every system captured once as a graph, replayed per cycle
-    if (!mc->graphExec || mc->graphSample != sample || mc->graphWpb != wpb)
+    if (!mc->graphExec || mc->graphSample != sample || mc->graphShape != shape)
     {
       if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
       if (mc->graph) (void)hipGraphDestroy(mc->graph);
       mc->graphExec = nullptr;
       mc->graph = nullptr;
       RG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const hipError_t le = launch(wpb);
+      const hipError_t le = launch(shape);
       hipError_t e = hipStreamEndCapture(ctx->stream, &mc->graph);
       if (le != hipSuccess) return hipfail(ctx, le, "k_gcmc launch under capture");
       if (e != hipSuccess) return hipfail(ctx, e, "hipStreamEndCapture");
       RG_HIP(ctx, hipGraphInstantiate(&mc->graphExec, mc->graph, nullptr, nullptr, 0));
       mc->graphSample = sample;
-      mc->graphWpb = wpb;
+      mc->graphShape = shape;
     }
     for (int c = done; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
   }
