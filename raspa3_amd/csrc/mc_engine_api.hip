@@ -316,7 +316,14 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
     return hipGetLastError();
   };
   Autotuner& tuner = Autotuner::instance();
-  const std::string key = "k_gcmc/systems=" + std::to_string(nsys) + "/cu=" + std::to_string(mc->computeUnits);
+  // the best shape depends on the loading (a team's barriers only pay off once the systems hold molecules), so the
+  // key carries the total adsorbate atom count rounded down to a power of two (host copy of the last run's count)
+  long long atoms = 0;
+  for (const rg_system* sy : mc->systems) atoms += sy->h.nads;
+  int bucket = 0;
+  while ((2ll << bucket) <= atoms) ++bucket;
+  const std::string key = "k_gcmc/systems=" + std::to_string(nsys) + "/cu=" + std::to_string(mc->computeUnits) +
+                          "/atoms>=" + std::to_string(atoms > 0 ? (1ll << bucket) : 0);
   int done = 0;
   for (int cand; mc->forcedShape < 0 && done < cycles && (cand = tuner.next(key, 4, 2)) >= 0; ++done)
   {
