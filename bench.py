@@ -26,6 +26,8 @@ Second half of the metric, in the same JSON line:
   300 K, samples/s and MFMA TFLOP/s at 1.08 MFLOP per training sample.
 * grid_check: the test the reference runs after building grids (1e5 random points per LJ grid, Boltzmann-weighted
   relative errors of the interpolated energy and gradient, and of the Ewald grid scaled by the probe charge).
+* gcmc_mfi: the same comparison in MFI (2 x 2 x 2 cells, the reference's silicalite test structure), proposal trained
+  at the simulation temperature; near saturation at 1e5 Pa.
 * gcmc_uneven: batched CBMC over 64 systems of uneven size and loading (ITQ-29 supercells of 1944-7200 sites),
   flexible n-hexane chains and rigid CO2 with the Ewald difference at kmax 8, one launch per cycle.
 """
@@ -279,8 +281,9 @@ def gcmc_stats(mc, nsys, base, temperature):
                 accepted_insertions=int(acc), insertion_trials=int(tri))
 
 
-def bench_gcmc(ctx, args):
-    """Plain CBMC against the learned first-bead proposal, CO2 in charged ITQ-29 at 300 K and 1e5 Pa."""
+def bench_gcmc(ctx, args, framework="itq29", tau=None):
+    """Plain CBMC against the learned first-bead proposal, CO2 at 300 K and 1e5 Pa in charged ITQ-29 (3 x 3 x 3
+    cells) or MFI (2 x 2 x 2 cells).  tau: training temperature factor of the proposal (default args.tau)."""
     import torch
 
     import raspa3_amd as ra
@@ -289,9 +292,15 @@ def bench_gcmc(ctx, args):
 
     T, P = 300.0, 1e5
     spec = w.forcefield_zeolite()
-    fw, unit = w.framework_itq29(spec, 3)
+    tau = args.tau if tau is None else tau
+    if framework == "mfi":
+        fw, unit = w.framework_mfi(spec, (2, 2, 2))
+        nuc, label = 288, "MFI 2x2x2 (2304 sites)"
+    else:
+        fw, unit = w.framework_itq29(spec, 3)
+        nuc, label = 72, "charged ITQ-29 3x3x3 (1944 sites)"
     ff = ra.ForceField(ctx, spec)
-    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw.pos[:72], fw.type[:72], fw.charge[:72]))
+    fu = ra.Framework(ctx, w.Framework(framework, unit, True, fw.pos[:nuc], fw.type[:nuc], fw.charge[:nuc]))
     dims = ra.grid_dims(unit, 0.2)
     types = [spec.index("C_co2"), spec.index("O_co2")]
     grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
@@ -330,7 +339,7 @@ def bench_gcmc(ctx, args):
         r.update(seconds=round(dt, 4), accepted_insertions_per_s=r["accepted_insertions"] / dt,
                  insertion_acceptance=r["accepted_insertions"] / max(1, r["insertion_trials"]),
                  kernel_ms_per_cycle=ev0.elapsed_time(ev1) / args.gcmc_production)
-        log(f"gcmc {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, N = {r['loading']:.3f} +- "
+        log(f"gcmc {framework} {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, N = {r['loading']:.3f} +- "
             f"{r['loading_err']:.3f}, Qst = {r['qst_kJmol']:.3f} +- {r['qst_err']:.3f} kJ/mol")
         mc.close()
         for s in systems:
@@ -350,7 +359,7 @@ def bench_gcmc(ctx, args):
     u_eff = np.where(np.isfinite(lw), -lw / beta, 1e7)
     net = ra.Mlp(ctx, seed=3)
     t = time.perf_counter()
-    losses = net.train(frac, u_eff, args.tau * beta, batch=4096, epochs=args.mlp_epochs, lr=1e-3, seed=9)
+    losses = net.train(frac, u_eff, tau * beta, batch=4096, epochs=args.mlp_epochs, lr=1e-3, seed=9)
     t_train = time.perf_counter() - t
     n3 = [args.voxels] * 3
     prob = net.proposal(n3, args.floor)
@@ -360,14 +369,14 @@ def bench_gcmc(ctx, args):
     def within(a, b, k):
         return abs(learned[a] - plain[a]) <= k * np.hypot(learned[b], plain[b])
 
-    return dict(workload=f"CO2 (reference rigid model) in charged ITQ-29 3x3x3 (1944 sites), {T:g} K, {P:g} Pa, "
+    return dict(workload=f"CO2 (reference rigid model) in {label}, {T:g} K, {P:g} Pa, "
                          f"{nsys} systems, {args.gcmc_equilibration} + {args.gcmc_production} cycles, grids at 0.2 A "
                          f"on the unit cell, Ewald kmax {list(kmax)}, 10 first-bead + 10 orientation trials",
                 plain=plain, learned=learned,
                 learned_over_plain=learned["accepted_insertions_per_s"] / plain["accepted_insertions_per_s"],
                 loading_within_2sigma=bool(within("loading", "loading_err", 2.0)),
                 qst_within_2sigma=bool(within("qst_kJmol", "qst_err", 2.0)),
-                proposal=dict(training_points=len(frac), orientations=16, tau=args.tau, floor=args.floor,
+                proposal=dict(training_points=len(frac), orientations=16, tau=tau, floor=args.floor,
                               voxels=n3, epochs=args.mlp_epochs, final_loss=float(losses[-1]),
                               data_seconds=round(t_data, 4), train_seconds=round(t_train, 4)),
                 unit="accepted insertions/s")
@@ -545,6 +554,7 @@ def main():
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)
         extra["gcmc"] = bench_gcmc(ctx, args)
+        extra["gcmc_mfi"] = bench_gcmc(ctx, args, "mfi", tau=1.0)
         extra["gcmc_uneven"] = bench_gcmc_uneven(ctx, args)
     base = None
     if not args.no_cpu_baseline:

@@ -253,6 +253,23 @@ def framework_itq29(spec, rep=3):
             np.diag([a, a, a]))
 
 
+def framework_mfi(spec, rep=(2, 2, 2)):
+    """MFI (silicalite-1: orthorhombic 20.022 x 19.899 x 13.383 A, 96 Si + 192 O per cell; a test structure of the
+    reference, Framework::makeMFI) repeated rep = (nx, ny, nz) times, with the force field's charges.  Returns
+    (framework, unit cell)."""
+    with open(os.path.join(DATA_DIR, "mfi.json")) as f:
+        d = json.load(f)
+    abc = np.array([d["a"], d["b"], d["c"]])
+    frac = np.array(d["fractional"], float)
+    typ = np.array([spec.index(e) for e in d["element"]], np.int32)
+    n = tuple(rep)
+    shifts = np.array([[i, j, k] for i in range(n[0]) for j in range(n[1]) for k in range(n[2])], float)
+    pos = ((frac[None, :, :] + shifts[:, None, :]) * abc).reshape(-1, 3)
+    t = np.tile(typ, len(shifts))
+    return (Framework("mfix%dx%dx%d" % n, np.diag(abc * np.array(n)), True, pos, t, spec.charge[t].astype(float)),
+            np.diag(abc))
+
+
 def co2_reference(spec, mol_fraction=1.0):
     """Component::makeCO2 (src/raspakit/component.cpp:2304-2329): O (0, 0, 1.149), C (0, 0, 0), O (0, 0, -1.149);
     first bead = C."""

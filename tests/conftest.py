@@ -32,6 +32,15 @@ def itq29():
 
 
 @pytest.fixture(scope="session")
+def mfi():
+    with open(os.path.join(GOLDEN, "mfi.json")) as f:
+        d = json.load(f)
+    abc = np.array([d["a"], d["b"], d["c"]])
+    frac = np.array(d["fractional"])
+    return dict(abc=abc, frac=frac, element=d["element"], charge=np.array(d["charge"]))
+
+
+@pytest.fixture(scope="session")
 def gpu_ctx():
     import raspa3_amd as ra
     ctx = ra.Context(0)

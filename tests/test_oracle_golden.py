@@ -155,3 +155,44 @@ def test_select_trial_exact():
     for u in np.linspace(0.0, 0.999999, 37):
         expect = int(np.searchsorted(c, u * c[-1], side="left"))
         assert oracle.select_trial(lbf, u) == expect
+
+
+def mfi_supercell(mfi, names, rep):
+    shifts = np.array([[i, j, k] for i in range(rep[0]) for j in range(rep[1]) for k in range(rep[2])], float)
+    pos = ((mfi["frac"][None, :, :] + shifts[:, None, :]) * mfi["abc"]).reshape(-1, 3)
+    t = np.tile(np.array([names.index(e) for e in mfi["element"]], np.int32), len(shifts))
+    return pos, t, np.tile(mfi["charge"], len(shifts)), mfi["abc"] * np.array(rep)
+
+
+def test_static_energy_co2_mfi(refvals, mfi):
+    """Two CO2 in MFI 2x2x2 (static_energy.cpp:48-73): pins the Pnma expansion of the reference's asymmetric unit
+    (tests/golden/make_fixtures.py) together with the pair terms."""
+    r = refvals["static_energy_co2_mfi"]
+    names, pairs = zeolite(refvals)
+    fpos, ftyp, fq, lengths = mfi_supercell(mfi, names, r["replicate"])
+    b = oracle.box(*lengths, lengths_only=True)
+    alpha, kmax = oracle.ewald_auto(1e-6, 12.0, b["widths"])
+    mols = co2(names, r["co2_positions"])
+    u = oracle.units()
+    e = oracle.framework_energy(b["cell"], True, fpos, ftyp, fq, pairs, 12.0, 12.0, alpha, True, mols)
+    assert abs(e[0] * u["energy_to_kelvin"] - r["framework_vdw_K"]) < TOL_K
+    assert abs(e[1] * u["energy_to_kelvin"] - r["framework_charge_K"]) < TOL_K
+    first = {k: v[:3] for k, v in mols.items()}
+    ei = oracle.intermolecular_energy(b["cell"], True, mols, pairs, 12.0, 12.0, alpha, True, first)
+    assert abs(ei[0] * u["energy_to_kelvin"] - r["molecule_vdw_K"]) < TOL_K
+    assert abs(ei[1] * u["energy_to_kelvin"] - r["molecule_charge_K"]) < TOL_K
+
+
+def test_ewald_two_co2_in_mfi(refvals, mfi):
+    """Single-ion and total Ewald Fourier energy of two CO2 in MFI 2x2x2 (ewald.cpp:248-280)."""
+    r = refvals["ewald_2co2_mfi"]
+    names, _ = zeolite(refvals)
+    fpos, ftyp, fq, lengths = mfi_supercell(mfi, names, r["replicate"])
+    b = oracle.box(*lengths, lengths_only=True)
+    u = oracle.units()
+    single = oracle.ewald_single_ion(b["cell"], True, r["alpha"], r["kmax"])
+    assert abs(single * u["energy_to_kelvin"] - r["single_ion_K"]) < TOL_K
+    m = co2(names, refvals["static_energy_co2_mfi"]["co2_positions"])
+    out, _ = oracle.ewald_total(b["cell"], True, r["alpha"], r["kmax"], fpos, fq, m["pos"], m["charge"], np.ones(6),
+                                m["molecule"])
+    assert abs(out.sum() * u["energy_to_kelvin"] - r["ewald_fourier_K"]) < TOL_K
