@@ -99,9 +99,18 @@ rg_status rg_framework_destroy(rg_framework* fw);
 
 /* ---------------------------------------------------------------------------------------------
  * Interpolation grids: InterpolationEnergyGrid (src/raspakit/interpolation_energy_grid.ixx:19-63),
- * tricubic scheme.  Device-resident, layout (8, nx, ny, nz) layout_left:
- * value l of point (i, j, k) at index l + 8*(i + nx*(j + ny*k)) (src/raspakit/interpolation_energy_grid.cpp:3807-3808).
+ * tricubic or triquintic scheme.  Device-resident, layout (V, nx, ny, nz) layout_left with V = 8 (tricubic)
+ * or 27 (triquintic): value l of point (i, j, k) at index l + V*(i + nx*(j + ny*k))
+ * (src/raspakit/interpolation_energy_grid.cpp:3807-3808, :3882-3924).
  * ------------------------------------------------------------------------------------------- */
+
+/* ForceField::InterpolationScheme (src/raspakit/forcefield/forcefield.ixx:79-84); the value is the number of
+ * stored values per grid point.  The Polynomial scheme is not provided. */
+typedef enum rg_interpolation_scheme
+{
+  RG_SCHEME_TRICUBIC = 8,
+  RG_SCHEME_TRIQUINTIC = 27
+} rg_interpolation_scheme;
 typedef enum rg_grid_type
 {
   RG_GRID_LENNARD_JONES = 0, /* ForceField::InterpolationGridType::LennardJones */
@@ -111,13 +120,20 @@ typedef enum rg_grid_type
 /* InterpolationEnergyGrid(unitCellBox, origin, numberOfGridPoints, Tricubic) —
  * src/raspakit/interpolation_energy_grid.cpp:3505-3514.  Allocates 8*nx*ny*nz doubles on the device. */
 rg_status rg_grid_create(rg_ctx* ctx, const double* cell9, int is_rectangular, int nx, int ny, int nz, rg_grid** out);
+/* InterpolationEnergyGrid(unitCellBox, origin, numberOfGridPoints, order) with the scheme given: allocates
+ * scheme*nx*ny*nz doubles (src/raspakit/interpolation_energy_grid.cpp:3505-3514). */
+rg_status rg_grid_create_scheme(rg_ctx* ctx, const double* cell9, int is_rectangular, int nx, int ny, int nz,
+                                rg_interpolation_scheme scheme, rg_grid** out);
 rg_status rg_grid_destroy(rg_grid* grid);
 /* Device address of the grid data (double*), for callers that keep it resident. */
 void* rg_grid_device_data(rg_grid* grid);
 
 /* InterpolationEnergyGrid::makeFrameworkInterpolationGrid(stream, gridType, forceField, framework, cutOff,
- * pseudo_atom_index), tricubic branch — src/raspakit/interpolation_energy_grid.cpp:3800-3875, per-point
- * analytic src/raspakit/interactions/framework_molecule_grid.cpp:168-282.  'cutoff_supercell' is the
+ * pseudo_atom_index) — src/raspakit/interpolation_energy_grid.cpp:3800-3965, per-point analytic
+ * src/raspakit/interactions/framework_molecule_grid.cpp:168-282 (tricubic) and :549-810 (triquintic; for a
+ * rectangular cell the values are the fractional derivatives themselves, where the reference's rectangular branch
+ * scales the y and z first derivatives by the cell's x length, :655-658 — equal for cubic cells).  The scheme is
+ * the grid's own.  'cutoff_supercell' is the
  * cutOff argument (it only sizes the supercell); pair inclusion uses the force field's framework VDW
  * cutoff, as in the reference.  Requires the supercell to span at least twice that VDW cutoff
  * (RG_ERR_UNSUPPORTED otherwise). */
@@ -140,11 +156,19 @@ rg_status rg_grid_build_many_async(rg_ctx* ctx, const rg_forcefield* ff, const r
 rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size_t count, double* host);
 rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count, const double* host);
 
-/* InterpolationEnergyGrid::interpolate (tricubic branch, src/raspakit/interpolation_energy_grid.cpp:3980-4070)
- * for n positions (pos = n*3, host).  energy[n] receives the interpolated energies; gradient (n*3) receives
- * d/dr (interpolateGradient, :4189-4347) when non-null. */
+/* InterpolationEnergyGrid::interpolate (tricubic and triquintic branches,
+ * src/raspakit/interpolation_energy_grid.cpp:3980-4180) for n positions (pos = n*3, host).  energy[n] receives the
+ * interpolated energies; gradient (n*3) receives d/dr (interpolateGradient, :4189-4440) when non-null. */
 rg_status rg_grid_interpolate(rg_ctx* ctx, const rg_grid* grid, int n, const double* pos, double* energy,
                               double* gradient);
+
+/* InterpolationEnergyGrid::interpolateHessian (src/raspakit/interpolation_energy_grid.cpp:4443-4760), either
+ * scheme: energies (n), Cartesian gradients (n*3) and Cartesian Hessians (n*9, row-major) of the interpolant.
+ * The Hessian is symmetric; the reference's triquintic branch fills its (z, y) entry with the (x, z) second
+ * derivative (:4748), and its tricubic branch returns a zero Hessian (:4600) where this returns the second
+ * derivatives of the tricubic interpolant. */
+rg_status rg_grid_interpolate_hessian(rg_ctx* ctx, const rg_grid* grid, int n, const double* pos, double* energy,
+                                      double* gradient, double* hessian);
 
 /* Test of a built grid on random points, as System::createFrameworkInterpolationGrids does after building
  * (src/raspakit/system_interpolation_grids.cpp:240-840, numberOfGridTestPoints = 1e5 by default): the tricubic

@@ -43,6 +43,12 @@ def lib(native=False):
         L.orc_energy_at_position.restype = C.c_double
         L.orc_interpolate.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, dp, dp]
         L.orc_tricubic_matrix.argtypes = [dp]
+        L.orc_grid_points_q.argtypes = grid_common + [C.c_int, ip, C.c_int, dp]
+        L.orc_grid_points_q.restype = C.c_int
+        L.orc_interpolate_hessian.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp]
+        L.orc_interpolate_q.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, dp, dp, dp]
+        L.orc_quintic_at_positions.argtypes = [dp, C.c_int, C.c_int, C.c_int, dp, ip, dp, C.c_int, dp, dp, dp, ip,
+                                               C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
         L.orc_grid_check.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip, dp, C.c_int, dp, dp,
                                      dp, ip, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp,
                                      C.c_double, C.c_double, C.c_int, dp, dp]
@@ -156,6 +162,51 @@ def grid_points(cell, rect, dims, grid_type, fw_pos, fw_type, fw_q, pairs, probe
                                      rc_supercell, alpha, overlap, len(ijk), _i(ijk), nthreads, _d(out))
     assert st == 0
     return out.reshape(-1, 8)
+
+
+def grid_points_q(cell, rect, dims, grid_type, fw_pos, fw_type, fw_q, pairs, probe, rc_vdw, rc_supercell, alpha,
+                  ijk, overlap=1e6, nthreads=1, native=False):
+    """Triquintic reference values (27 per point, scaled and clamped as stored) at grid points ijk (m, 3)."""
+    ijk = _n(ijk).reshape(-1, 3)
+    out = np.zeros(27 * len(ijk))
+    c9, r, na, p, t, q, nt, e, s, sh, lj = _fw_args(cell, rect, fw_pos, fw_type, fw_q, pairs)
+    st = lib(native).orc_grid_points_q(c9, r, *dims, grid_type, na, p, t, q, nt, e, s, sh, lj, probe, rc_vdw,
+                                       rc_supercell, alpha, overlap, len(ijk), _i(ijk), nthreads, _d(out))
+    assert st == 0
+    return out.reshape(-1, 27)
+
+
+def quintic_at_positions(cell, rect, grid_type, fw_pos, fw_type, fw_q, pairs, probe, rc_vdw, rc_supercell, alpha,
+                         pos):
+    """The 27 unscaled fractional derivatives of the framework energy at Cartesian positions (m, 3)."""
+    pos = _f(pos).reshape(-1, 3)
+    out = np.zeros(27 * len(pos))
+    c9, r, na, p, t, q, nt, e, s, sh, lj = _fw_args(cell, rect, fw_pos, fw_type, fw_q, pairs)
+    lib().orc_quintic_at_positions(c9, r, grid_type, na, p, t, q, nt, e, s, sh, lj, probe, rc_vdw, rc_supercell,
+                                   alpha, len(pos), _d(pos), _d(out))
+    return out.reshape(-1, 27)
+
+
+def interpolate_hessian(cell, rect, dims, grid, pos):
+    """Tricubic interpolation of a (nz, ny, nx, 8) grid: energies (m,), gradients (m, 3), Hessians (m, 3, 3)."""
+    pos = _f(pos).reshape(-1, 3)
+    e = np.zeros(len(pos))
+    g = np.zeros((len(pos), 3))
+    h = np.zeros((len(pos), 3, 3))
+    lib().orc_interpolate_hessian(_d(_f(np.asarray(cell).T.reshape(9))), int(rect), *dims, _d(_f(grid).reshape(-1)),
+                                  len(pos), _d(pos), _d(e), _d(g), _d(h))
+    return e, g, h
+
+
+def interpolate_q(cell, rect, dims, grid, pos):
+    """Triquintic interpolation of a (nz, ny, nx, 27) grid: energies (m,), gradients (m, 3), Hessians (m, 3, 3)."""
+    pos = _f(pos).reshape(-1, 3)
+    e = np.zeros(len(pos))
+    g = np.zeros((len(pos), 3))
+    h = np.zeros((len(pos), 3, 3))
+    lib().orc_interpolate_q(_d(_f(np.asarray(cell).T.reshape(9))), int(rect), *dims, _d(_f(grid).reshape(-1)),
+                            len(pos), _d(pos), _d(e), _d(g), _d(h))
+    return e, g, h
 
 
 def energy_at_position(cell, rect, grid_type, fw_pos, fw_type, fw_q, pairs, probe, rc_vdw, rc_supercell, alpha, p):

@@ -24,6 +24,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -390,7 +391,7 @@ const TricubicMatrix TM;
 // src/raspakit/interpolation_energy_grid.cpp:3980-4070 (tricubic branch of interpolate) and :4189-4347
 // (gradient): s = fract(H^-1 pos), cell index = min(floor(s*nc), nc-1), gather, a = A X, polynomial.
 void interpolateOne(const Box& unit, const int np[3], const double* data, const double pos[3], double* e,
-                    double grad[3])
+                    double grad[3], double* hess = nullptr)
 {
   const int nc[3] = {np[0] - 1, np[1] - 1, np[2] - 1};
   double s[3];
@@ -426,6 +427,15 @@ void interpolateOne(const Box& unit, const int np[3], const double* data, const 
     a[r] = acc;
   }
   double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  // p(x, n, d): d-th derivative of x^n
+  auto p = [](double x, int n, int d)
+  {
+    if (d > n) return 0.0;
+    double f = 1.0;
+    for (int m = 0; m < d; ++m) f *= double(n - m);
+    return f * std::pow(x, n - d);
+  };
   for (int k = 0; k < 4; ++k)
     for (int j = 0; j < 4; ++j)
       for (int i = 0; i < 4; ++i)
@@ -439,8 +449,32 @@ void interpolateOne(const Box& unit, const int np[3], const double* data, const 
           if (j > 0) gy += j * c * px * std::pow(t[1], j - 1) * pz;
           if (k > 0) gz += k * c * px * py * std::pow(t[2], k - 1);
         }
+        if (hess)
+        {
+          const int ijk[3] = {i, j, k};
+          for (int a1 = 0; a1 < 3; ++a1)
+            for (int a2 = 0; a2 < 3; ++a2)
+            {
+              double term = c;
+              for (int ax = 0; ax < 3; ++ax) term *= p(t[ax], ijk[ax], int(ax == a1) + int(ax == a2));
+              H[a1][a2] += term;
+            }
+        }
       }
   *e = v;
+  if (hess)
+  {
+    // d2E/dr_i dr_j = sum_ab (H^-1)_ai nc_a nc_b d2E/dt_a dt_b (H^-1)_bj
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+      {
+        double acc = 0.0;
+        for (int a1 = 0; a1 < 3; ++a1)
+          for (int a2 = 0; a2 < 3; ++a2)
+            acc += unit.inv.c[i][a1] * H[a1][a2] * double(nc[a1]) * double(nc[a2]) * unit.inv.c[j][a2];
+        hess[3 * i + j] = acc;
+      }
+  }
   if (grad)
   {
     // d/ds = nc * d/dt ; d/dr = H^-T d/ds
@@ -799,6 +833,16 @@ void orc_interpolate(const double* cell9, int rect, int nx, int ny, int nz, cons
   Box unit = makeBox(cell9, rect);
   const int np[3] = {nx, ny, nz};
   for (int q = 0; q < n; ++q) interpolateOne(unit, np, grid, pos + 3 * q, e + q, grad ? grad + 3 * q : nullptr);
+}
+
+// Tricubic energies, gradients and Hessians (n*9, row-major, Cartesian) of the interpolant.  The reference's
+// interpolateHessian returns a zero Hessian for this scheme (src/raspakit/interpolation_energy_grid.cpp:4600).
+void orc_interpolate_hessian(const double* cell9, int rect, int nx, int ny, int nz, const double* grid, int n,
+                             const double* pos, double* e, double* grad, double* hess)
+{
+  Box unit = makeBox(cell9, rect);
+  const int np[3] = {nx, ny, nz};
+  for (int q = 0; q < n; ++q) interpolateOne(unit, np, grid, pos + 3 * q, e + q, grad + 3 * q, hess + 9 * q);
 }
 
 // The grid test of System::createFrameworkInterpolationGrids (src/raspakit/system_interpolation_grids.cpp:240-840,
@@ -1271,3 +1315,4 @@ int orc_select_trial(int n, const double* lbf, double uniform)
 }  // extern "C"
 
 #include "oracle_mc.inc"
+#include "oracle_quintic.inc"

@@ -157,23 +157,26 @@ class Framework(_Handle):
 
 
 class Grid(_Handle):
-    """rg_grid: a device-resident tricubic grid (8 values per point, layout_left)."""
+    """rg_grid: a device-resident interpolation grid, layout_left (values, nx, ny, nz); scheme 8 = tricubic,
+    27 = triquintic (ForceField::InterpolationScheme)."""
 
     _destroy = "rg_grid_destroy"
 
-    def __init__(self, ctx, cell, rect, n):
+    def __init__(self, ctx, cell, rect, n, scheme=8):
         self.ctx = ctx
         self.cell = np.asarray(cell, float)
         self.rect = bool(rect)
         self.n = tuple(int(x) for x in n)
+        self.values = int(scheme)
         h = C.c_void_p()
         c9 = L.f64(self.cell.T.reshape(9))
-        L.check(lib().rg_grid_create(ctx.h, L.d(c9), int(rect), *self.n, C.byref(h)), ctx.h, "rg_grid_create")
+        L.check(lib().rg_grid_create_scheme(ctx.h, L.d(c9), int(rect), *self.n, self.values, C.byref(h)), ctx.h,
+                "rg_grid_create_scheme")
         self._register(ctx, h)
 
     @property
     def count(self):
-        return 8 * self.n[0] * self.n[1] * self.n[2]
+        return self.values * self.n[0] * self.n[1] * self.n[2]
 
     def download(self, offset=0, count=None):
         count = self.count - offset if count is None else count
@@ -192,6 +195,16 @@ class Grid(_Handle):
         L.check(lib().rg_grid_interpolate(self.ctx.h, self.h, len(pos), L.d(pos), L.d(e), L.d(g)), self.ctx.h,
                 "rg_grid_interpolate")
         return (e, g) if gradient else e
+
+    def interpolate_hessian(self, pos):
+        """InterpolationEnergyGrid::interpolateHessian: energies (m,), gradients (m, 3), Hessians (m, 3, 3)."""
+        pos = L.f64(pos).reshape(-1, 3)
+        e = np.empty(len(pos))
+        g = np.empty((len(pos), 3))
+        h = np.empty((len(pos), 3, 3))
+        L.check(lib().rg_grid_interpolate_hessian(self.ctx.h, self.h, len(pos), L.d(pos), L.d(e), L.d(g), L.d(h)),
+                self.ctx.h, "rg_grid_interpolate_hessian")
+        return e, g, h
 
 
 

@@ -21,6 +21,7 @@ vp = C.c_void_p
 
 RG_OK, RG_ERR_INVALID, RG_ERR_UNSUPPORTED, RG_ERR_DEVICE, RG_ERR_NOMEM = range(5)
 RG_GRID_LENNARD_JONES, RG_GRID_EWALD_REAL = 0, 3
+RG_SCHEME_TRICUBIC, RG_SCHEME_TRIQUINTIC = 8, 27
 
 # name -> (restype, argtypes); every symbol declared in include/raspa3_amd/rg.h
 SIGNATURES = {
@@ -39,6 +40,7 @@ SIGNATURES = {
     "rg_framework_create": (C.c_int, [vp, dp, C.c_int, C.c_int, dp, ip, dp, C.POINTER(vp)]),
     "rg_framework_destroy": (C.c_int, [vp]),
     "rg_grid_create": (C.c_int, [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    "rg_grid_create_scheme": (C.c_int, [vp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
     "rg_grid_destroy": (C.c_int, [vp]),
     "rg_grid_device_data": (vp, [vp]),
     "rg_grid_build": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp]),
@@ -47,6 +49,7 @@ SIGNATURES = {
     "rg_grid_download": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
     "rg_grid_upload": (C.c_int, [vp, vp, C.c_size_t, C.c_size_t, dp]),
     "rg_grid_interpolate": (C.c_int, [vp, vp, C.c_int, dp, dp, dp]),
+    "rg_grid_interpolate_hessian": (C.c_int, [vp, vp, C.c_int, dp, dp, dp, dp]),
     "rg_ewald_parameters": (C.c_int, [C.c_double, C.c_double, dp, dp, ip]),
     "rg_system_create": (C.c_int, [vp, vp, dp, C.c_int, vp, C.c_int, ip, C.POINTER(vp), vp, ip, C.c_longlong,
                                    C.c_int, C.POINTER(vp)]),

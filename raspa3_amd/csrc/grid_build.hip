@@ -28,6 +28,7 @@
 #include <cstring>
 
 #include "ewald_tables.hpp"
+#include "quintic_terms.hpp"
 #include "mc_device.hpp"
 #include "rg_internal.hpp"
 
@@ -59,7 +60,7 @@ struct GridArgs
   double cell[9];
   double delta[3];
   double ncm1[3];
-  double G01, G02, G12;
+  double G00, G11, G22, G01, G02, G12;
   double diag[3];
   double rc2;
   double cut2;
@@ -192,20 +193,28 @@ struct Acc
   }
 };
 
-template <int NLJ, bool COUL, bool TRIC>
-__global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
-{
-  constexpr int NG = NLJ + (COUL ? 1 : 0);
-  __shared__ Img cand[CH];
-  __shared__ LJPar par[NLJ > 0 ? NLJ : 1][MAXT];
-  __shared__ int rowStart[MAXROWS];
-  __shared__ int rowPrefix[MAXROWS + 1];
-  __shared__ int waveCount[4];
-  __shared__ __attribute__((aligned(16))) double erfcxTab[COUL ? kErfcxTabLen : 1];
+// ---- tile set-up and candidate staging, shared by the tricubic and triquintic kernels
 
+// The lane's grid point and its tile.
+struct TilePoint
+{
+  int gi, gj, gk;
+  bool active;
+  double px, py, pz;     // Cartesian position of the point
+  double pu0, pu1, pu2;  // h_a . p (triclinic only)
+  double cxc, cyc, czc;  // centre of the tile's points
+  int nrow;              // rows of bins in the tile's bin box
+};
+
+// Finds the lane's point and fills the tile's bin-row table: rowStart[r] = first image of row r, rowPrefix[r] =
+// images in rows before r (rowPrefix[nrow] = total).  Ends with a barrier.
+template <bool TRIC>
+__device__ __forceinline__ TilePoint tileSetup(const GridArgs& A, int* rowStart, int* rowPrefix)
+{
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wv = tid >> 6;
+  TilePoint T;
 
   // XCD-aware tile order: hardware places workgroup b on XCD b % 8; give each XCD a contiguous range of tiles.
   int bid = blockIdx.x;
@@ -217,31 +226,21 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   const int ty = (bid / A.ntile[0]) % A.ntile[1];
   const int tz = bid / (A.ntile[0] * A.ntile[1]);
 
-  if (NLJ > 0)
-  {
-    for (int q = tid; q < NLJ * MAXT; q += 256) (&par[0][0])[q] = A.ljpar[q];
-  }
-  if (COUL)
-  {
-    for (int q = tid; q < kErfcxTabLen; q += 256) erfcxTab[q] = erfcxTabEntry(q);
-  }
-
-  // this lane's grid point
-  const int gi = tx * TI + (wv & 1) * 4 + (lane & 3);
-  const int gj = ty * TJ + (wv >> 1) * 4 + ((lane >> 2) & 3);
-  const int gk = tz * TK + (lane >> 4);
-  const bool active = gi < A.n[0] && gj < A.n[1] && gk < A.n[2];
+  T.gi = tx * TI + (wv & 1) * 4 + (lane & 3);
+  T.gj = ty * TJ + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  T.gk = tz * TK + (lane >> 4);
+  T.active = T.gi < A.n[0] && T.gj < A.n[1] && T.gk < A.n[2];
   // s = (i*delta_x, j*delta_y, k*delta_z); pos = cell * s (src/raspakit/interpolation_energy_grid.cpp:3829-3831)
-  const double s0 = double(gi) * A.delta[0], s1 = double(gj) * A.delta[1], s2 = double(gk) * A.delta[2];
-  const double px = A.cell[0] * s0 + A.cell[3] * s1 + A.cell[6] * s2;
-  const double py = A.cell[1] * s0 + A.cell[4] * s1 + A.cell[7] * s2;
-  const double pz = A.cell[2] * s0 + A.cell[5] * s1 + A.cell[8] * s2;
-  double pu0 = 0.0, pu1 = 0.0, pu2 = 0.0;
+  const double s0 = double(T.gi) * A.delta[0], s1 = double(T.gj) * A.delta[1], s2 = double(T.gk) * A.delta[2];
+  T.px = A.cell[0] * s0 + A.cell[3] * s1 + A.cell[6] * s2;
+  T.py = A.cell[1] * s0 + A.cell[4] * s1 + A.cell[7] * s2;
+  T.pz = A.cell[2] * s0 + A.cell[5] * s1 + A.cell[8] * s2;
+  T.pu0 = T.pu1 = T.pu2 = 0.0;
   if (TRIC)
   {
-    pu0 = A.cell[0] * px + A.cell[1] * py + A.cell[2] * pz;
-    pu1 = A.cell[3] * px + A.cell[4] * py + A.cell[5] * pz;
-    pu2 = A.cell[6] * px + A.cell[7] * py + A.cell[8] * pz;
+    T.pu0 = A.cell[0] * T.px + A.cell[1] * T.py + A.cell[2] * T.pz;
+    T.pu1 = A.cell[3] * T.px + A.cell[4] * T.py + A.cell[5] * T.pz;
+    T.pu2 = A.cell[6] * T.px + A.cell[7] * T.py + A.cell[8] * T.pz;
   }
 
   // centre of the tile's points (fractional and Cartesian; a tile at the grid edge holds fewer points) and the
@@ -249,9 +248,9 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   const double fc0 = 0.5 * double(2 * tx * TI + min(TI, A.n[0] - tx * TI) - 1) * A.delta[0];
   const double fc1 = 0.5 * double(2 * ty * TJ + min(TJ, A.n[1] - ty * TJ) - 1) * A.delta[1];
   const double fc2 = 0.5 * double(2 * tz * TK + min(TK, A.n[2] - tz * TK) - 1) * A.delta[2];
-  const double cxc = A.cell[0] * fc0 + A.cell[3] * fc1 + A.cell[6] * fc2;
-  const double cyc = A.cell[1] * fc0 + A.cell[4] * fc1 + A.cell[7] * fc2;
-  const double czc = A.cell[2] * fc0 + A.cell[5] * fc1 + A.cell[8] * fc2;
+  T.cxc = A.cell[0] * fc0 + A.cell[3] * fc1 + A.cell[6] * fc2;
+  T.cyc = A.cell[1] * fc0 + A.cell[4] * fc1 + A.cell[7] * fc2;
+  T.czc = A.cell[2] * fc0 + A.cell[5] * fc1 + A.cell[8] * fc2;
   int blo[3], bhi[3];
   {
     const double fc[3] = {fc0, fc1, fc2};
@@ -266,6 +265,7 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   }
   const int n1r = bhi[1] - blo[1] + 1;
   const int nrow = n1r * (bhi[2] - blo[2] + 1);
+  T.nrow = nrow;
 
   // rows of bins contiguous along axis 0: [start(blo0, by, bz), start(bhi0 + 1, by, bz))
   for (int r = tid; r < nrow; r += 256)
@@ -297,60 +297,99 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     }
   }
   __syncthreads();
-  const int total = rowPrefix[nrow];
+  return T;
+}
+
+// Gathers images [base, base + 256) of the tile's bin rows, keeps those within the tile's sphere that matter to the
+// launch (charged for the Coulomb grid, Lennard-Jones-active for one of the NLJ probes), compacts them into cand
+// with the charge scaled by the Coulomb factor, and returns their number.  Every thread of the workgroup calls it;
+// it ends with a barrier.
+template <int NLJ, bool COUL>
+__device__ __forceinline__ int stageChunk(const GridArgs& A, const TilePoint& T, int base, int total,
+                                          const int* rowStart, const int* rowPrefix, const LJPar (*par)[MAXT],
+                                          Img* cand, int* waveCount)
+{
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int g = base + tid;
+  bool pass = false;
+  double2 r0, r1, r2, r3;  // the image record as (x, y), (z, q), (u0, u1), (u2, type)
+  if (g < total)
+  {
+    int lo = 0, hi = T.nrow - 1;
+    while (lo < hi)
+    {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rowPrefix[mid] <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const double2* src = reinterpret_cast<const double2*>(A.img + rowStart[lo] + (g - rowPrefix[lo]));
+    r0 = src[0];
+    r1 = src[1];
+    r2 = src[2];
+    r3 = src[3];
+    const int type = static_cast<int>(__double_as_longlong(r3.y) & 0xffffffffll);
+    const double dx = r0.x - T.cxc, dy = r0.y - T.cyc, dz = r1.x - T.czc;
+    pass = dx * dx + dy * dy + dz * dz < A.cut2;
+    bool useful = COUL && r1.y != 0.0;
+#pragma unroll
+    for (int q = 0; q < NLJ; ++q) useful = useful || par[q][type].e4 != 0.0;
+    pass = pass && useful;
+  }
+  const unsigned long long m = __ballot(pass);
+  if (lane == 0) waveCount[wv] = __popcll(m);
+  __syncthreads();
+  int off = 0;
+  for (int q = 0; q < wv; ++q) off += waveCount[q];
+  const int ncand = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+  if (pass)
+  {
+    r1.y *= A.coulomb;
+    double2* dst = reinterpret_cast<double2*>(&cand[off + __popcll(m & lt_mask)]);
+    dst[0] = r0;
+    dst[1] = r1;
+    dst[2] = r2;
+    dst[3] = r3;
+  }
+  __syncthreads();
+  return ncand;
+}
+
+template <int NLJ, bool COUL, bool TRIC>
+__global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
+{
+  constexpr int NG = NLJ + (COUL ? 1 : 0);
+  __shared__ Img cand[CH];
+  __shared__ LJPar par[NLJ > 0 ? NLJ : 1][MAXT];
+  __shared__ int rowStart[MAXROWS];
+  __shared__ int rowPrefix[MAXROWS + 1];
+  __shared__ int waveCount[4];
+  __shared__ __attribute__((aligned(16))) double erfcxTab[COUL ? kErfcxTabLen : 1];
+
+  const int tid = threadIdx.x;
+  if (NLJ > 0)
+  {
+    for (int q = tid; q < NLJ * MAXT; q += 256) (&par[0][0])[q] = A.ljpar[q];
+  }
+  if (COUL)
+  {
+    for (int q = tid; q < kErfcxTabLen; q += 256) erfcxTab[q] = erfcxTabEntry(q);
+  }
+  const TilePoint T = tileSetup<TRIC>(A, rowStart, rowPrefix);
+  const int total = rowPrefix[T.nrow];
+  const double px = T.px, py = T.py, pz = T.pz;
 
   Acc<TRIC> acc[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) acc[g].zero();
 
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-
   for (int base = 0; base < total; base += 256)
   {
-    // ---- gather + sphere filter + compaction into LDS
-    const int g = base + tid;
-    bool pass = false;
-    double2 r0, r1, r2, r3;  // the image record as (x, y), (z, q), (u0, u1), (u2, type)
-    if (g < total)
-    {
-      int lo = 0, hi = nrow - 1;
-      while (lo < hi)
-      {
-        const int mid = (lo + hi + 1) >> 1;
-        if (rowPrefix[mid] <= g)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      const double2* src = reinterpret_cast<const double2*>(A.img + rowStart[lo] + (g - rowPrefix[lo]));
-      r0 = src[0];
-      r1 = src[1];
-      r2 = src[2];
-      r3 = src[3];
-      const int type = static_cast<int>(__double_as_longlong(r3.y) & 0xffffffffll);
-      const double dx = r0.x - cxc, dy = r0.y - cyc, dz = r1.x - czc;
-      pass = dx * dx + dy * dy + dz * dz < A.cut2;
-      bool useful = COUL && r1.y != 0.0;
-#pragma unroll
-      for (int q = 0; q < NLJ; ++q) useful = useful || par[q][type].e4 != 0.0;
-      pass = pass && useful;
-    }
-    const unsigned long long m = __ballot(pass);
-    if (lane == 0) waveCount[wv] = __popcll(m);
-    __syncthreads();
-    int off = 0;
-    for (int q = 0; q < wv; ++q) off += waveCount[q];
-    const int ncand = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
-    if (pass)
-    {
-      r1.y *= A.coulomb;
-      double2* dst = reinterpret_cast<double2*>(&cand[off + __popcll(m & lt_mask)]);
-      dst[0] = r0;
-      dst[1] = r1;
-      dst[2] = r2;
-      dst[3] = r3;
-    }
-    __syncthreads();
+    const int ncand = stageChunk<NLJ, COUL>(A, T, base, total, rowStart, rowPrefix, par, cand, waveCount);
 
     // ---- per-lane pair sums over the staged candidates; the position and charge of the next candidate are read
     // while the current one is summed
@@ -373,9 +412,9 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
         double u0, u1, u2;
         if (TRIC)
         {
-          u0 = pu0 - b.u0;
-          u1 = pu1 - b.u1;
-          u2 = pu2 - b.u2;
+          u0 = T.pu0 - b.u0;
+          u1 = T.pu1 - b.u1;
+          u2 = T.pu2 - b.u2;
         }
         else
         {
@@ -432,8 +471,8 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     __syncthreads();
   }
 
-  if (!active) return;
-  const size_t pidx = 8 * (size_t(gi) + size_t(A.n[0]) * (size_t(gj) + size_t(A.n[1]) * size_t(gk)));
+  if (!T.active) return;
+  const size_t pidx = 8 * (size_t(T.gi) + size_t(A.n[0]) * (size_t(T.gj) + size_t(A.n[1]) * size_t(T.gk)));
 #pragma unroll
   for (int q = 0; q < NG; ++q)
   {
@@ -482,6 +521,140 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
     o[2] = make_double2(v[4], v[5]);
     o[3] = make_double2(v[6], v[7]);
   }
+}
+
+// ---- triquintic builder: one grid per launch (Lennard-Jones of one probe, or the unit-charge Ewald real-space grid).
+// Per pair the radial factors f_0 .. f_6 (f_k = ((1/r) d/dr)^k U) are folded into the moments S(k, e) = sum f_k u^e
+// of quintic_terms.hpp; the 27 values are linear combinations of the moments, formed once per point.
+//   LJ:    src/raspakit/potentials/interpolation/triquintic_derivative_lj.ixx:33-53
+//   Ewald: src/raspakit/potentials/interpolation/triquintic_derivative_real_ewald.ixx:33-80
+template <bool LJ, bool TRIC>
+__global__ __launch_bounds__(256) void k_grid_build_q(GridArgs A)
+{
+  constexpr int NLJ = LJ ? 1 : 0;
+  constexpr bool COUL = !LJ;
+  constexpr int NM = TRIC ? rgq::kMomFull : rgq::kMomDiag;
+  __shared__ Img cand[CH];
+  __shared__ LJPar par[1][MAXT];
+  __shared__ int rowStart[MAXROWS];
+  __shared__ int rowPrefix[MAXROWS + 1];
+  __shared__ int waveCount[4];
+  __shared__ __attribute__((aligned(16))) double erfcxTab[COUL ? kErfcxTabLen : 1];
+
+  const int tid = threadIdx.x;
+  if (LJ)
+  {
+    for (int q = tid; q < MAXT; q += 256) par[0][q] = A.ljpar[q];
+  }
+  else
+  {
+    for (int q = tid; q < kErfcxTabLen; q += 256) erfcxTab[q] = erfcxTabEntry(q);
+  }
+  const TilePoint T = tileSetup<TRIC>(A, rowStart, rowPrefix);
+  const int total = rowPrefix[T.nrow];
+
+  double S[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) S[m] = 0.0;
+
+  for (int base = 0; base < total; base += 256)
+  {
+    const int ncand = stageChunk<NLJ, COUL>(A, T, base, total, rowStart, rowPrefix, par, cand, waveCount);
+    const double2* cb = reinterpret_cast<const double2*>(cand);
+    double2 nx0 = cb[0], nx1 = cb[1];
+    for (int c = 0; c < ncand; ++c)
+    {
+      const double2 c0 = nx0, c1 = nx1;  // (x, y), (z, q)
+      {
+        const int cn = min(c + 1, ncand - 1);
+        nx0 = cb[4 * cn];
+        nx1 = cb[4 * cn + 1];
+      }
+      const Img& b = cand[c];
+      const double dx = T.px - c0.x, dy = T.py - c0.y, dz = T.pz - c1.x;
+      double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < A.rc2)
+      {
+        rr = fmax(rr, 0.1);  // src/raspakit/interactions/framework_molecule_grid.cpp:576
+        double f[7];
+        if (LJ)
+        {
+          const LJPar& P = par[0][b.type];
+          const double ir2 = rcp64(rr);
+          const double x = P.s2 * ir2;
+          const double t3 = x * x * x;
+          const double a6 = P.e4 * (t3 * t3), a3 = P.e4 * t3;  // 4 eps t^6, 4 eps t^3
+          const double ir4 = ir2 * ir2, ir6 = ir4 * ir2, ir8 = ir4 * ir4, ir10 = ir8 * ir2, ir12 = ir6 * ir6;
+          f[0] = (a6 - a3) - P.sh;
+          f[1] = -6.0 * fma(2.0, a6, -a3) * ir2;
+          f[2] = 24.0 * fma(7.0, a6, -2.0 * a3) * ir4;
+          f[3] = -96.0 * fma(28.0, a6, -5.0 * a3) * ir6;
+          f[4] = 1152.0 * fma(42.0, a6, -5.0 * a3) * ir8;
+          f[5] = -80640.0 * fma(12.0, a6, -a3) * ir10;
+          f[6] = 645120.0 * fma(33.0, a6, -2.0 * a3) * ir12;
+        }
+        else
+        {
+          // unit probe charge; the staged charge carries the Coulomb conversion factor
+          const double ir = rsq64(rr);
+          const double r = rr * ir;
+          const double ir2 = ir * ir;
+          const double ar2 = A.alpha2 * rr;
+          const double qex = c1.y * expNeg(ar2);                   // q exp(-alpha^2 r^2)
+          const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);  // q erfc(alpha r)
+          const double etr = A.k2a * r * qex;                     // q 2 alpha / sqrt(pi) r exp(-alpha^2 r^2)
+          const double p2 = fma(2.0, ar2, 3.0);
+          const double p3 = fma(ar2, fma(4.0, ar2, 10.0), 15.0);
+          const double p4 = fma(ar2, fma(ar2, fma(8.0, ar2, 28.0), 70.0), 105.0);
+          const double p5 = fma(ar2, fma(ar2, fma(ar2, fma(16.0, ar2, 72.0), 252.0), 630.0), 945.0);
+          const double p6 =
+              fma(ar2, fma(ar2, fma(ar2, fma(ar2, fma(32.0, ar2, 176.0), 792.0), 2772.0), 6930.0), 10395.0);
+          const double ir3 = ir2 * ir, ir5 = ir3 * ir2, ir7 = ir5 * ir2, ir9 = ir7 * ir2, ir11 = ir9 * ir2,
+                       ir13 = ir11 * ir2;
+          f[0] = ec * ir;
+          f[1] = -(etr + ec) * ir3;
+          f[2] = fma(etr, p2, 3.0 * ec) * ir5;
+          f[3] = -fma(etr, p3, 15.0 * ec) * ir7;
+          f[4] = fma(etr, p4, 105.0 * ec) * ir9;
+          f[5] = -fma(etr, p5, 945.0 * ec) * ir11;
+          f[6] = fma(etr, p6, 10395.0 * ec) * ir13;
+        }
+        if (TRIC)
+          rgq::accumulate<true>(S, f, T.pu0 - b.u0, T.pu1 - b.u1, T.pu2 - b.u2);
+        else
+          rgq::accumulate<false>(S, f, dx, dy, dz);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (!T.active) return;
+  double v[27];
+  {
+    const double G[6] = {A.G00, A.G11, A.G22, A.G01, A.G02, A.G12};
+    rgq::combine<TRIC>(S, G, v);
+  }
+  // rectangular cells: Cartesian derivatives times the cell lengths (framework_molecule_grid.cpp:655-689); then
+  // the cell counts (src/raspakit/interpolation_energy_grid.cpp:3888-3924)
+#pragma unroll
+  for (int o = 1; o < 27; ++o)
+  {
+    double sc = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      for (int r = 0; r < rgq::kOrders[o][a]; ++r) sc *= TRIC ? 1.0 / A.ncm1[a] : A.diag[a] / A.ncm1[a];
+    v[o] *= sc;
+  }
+  // src/raspakit/interpolation_energy_grid.cpp:3926-3956
+  if (v[0] > A.overlap)
+  {
+    v[0] = 2.0 * A.overlap;
+    for (int l = 1; l < 4; ++l) v[l] = fmin(fmax(v[l], -A.overlap), A.overlap);
+    for (int l = 4; l < 27; ++l) v[l] = 0.0;
+  }
+  double* o = A.out[0] + 27 * (size_t(T.gi) + size_t(A.n[0]) * (size_t(T.gj) + size_t(A.n[1]) * size_t(T.gk)));
+#pragma unroll
+  for (int l = 0; l < 27; ++l) o[l] = v[l];
 }
 
 using KernelFn = void (*)(GridArgs);
@@ -634,6 +807,8 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
     if (!G || !G->d) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: grid %d is null", g);
     if (G->n[0] != g0->n[0] || G->n[1] != g0->n[1] || G->n[2] != g0->n[2])
       return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: fused grids must have equal dimensions");
+    if (G->values != g0->values)
+      return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: fused grids must share the interpolation scheme");
     if (std::memcmp(G->cell, fw->cell, sizeof(G->cell)) != 0)
       return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: grid %d cell differs from the framework unit cell", g);
     if (types[g] != RG_GRID_LENNARD_JONES && types[g] != RG_GRID_EWALD_REAL)
@@ -704,6 +879,9 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
     A.bw[a] = fw->images.bw[a];
     A.nb[a] = fw->images.nb[a];
   }
+  A.G00 = H[0] * H[0] + H[1] * H[1] + H[2] * H[2];
+  A.G11 = H[3] * H[3] + H[4] * H[4] + H[5] * H[5];
+  A.G22 = H[6] * H[6] + H[7] * H[7] + H[8] * H[8];
   A.G01 = H[0] * H[3] + H[1] * H[4] + H[2] * H[5];
   A.G02 = H[0] * H[6] + H[1] * H[7] + H[2] * H[8];
   A.G12 = H[3] * H[6] + H[4] * H[7] + H[5] * H[8];
@@ -729,6 +907,39 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
   A.overlap = ff->overlap;
 
   const bool tric = !fw->rect;
+  if (g0->values == 27)
+  {
+    // triquintic: one launch per grid, the LJ parameter rows of launch l at table offset l * MAXT
+    std::vector<LJPar> table(size_t(n_grids) * MAXT, LJPar{});
+    for (int g = 0; g < n_grids; ++g)
+      if (types[g] == RG_GRID_LENNARD_JONES)
+        for (size_t row = 0; row < rowType.size(); ++row)
+          table[size_t(g) * MAXT + row] = ljPar(ff, probes[g], rowType[row]);
+    const size_t bytes = table.size() * sizeof(LJPar);
+    if (ctx->table_bytes < bytes)
+    {
+      RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (ctx->d_table) hipFree(ctx->d_table);
+      ctx->d_table = nullptr;
+      ctx->table_bytes = 0;
+      RG_HIP(ctx, hipMalloc(&ctx->d_table, bytes));
+      ctx->table_bytes = bytes;
+    }
+    RG_HIP(ctx, hipMemcpyAsync(ctx->d_table, table.data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    for (int g = 0; g < n_grids; ++g)
+    {
+      const bool isLJ = types[g] == RG_GRID_LENNARD_JONES;
+      for (int q = 0; q <= MAXG; ++q) A.out[q] = nullptr;
+      A.out[0] = grids[g]->d;
+      A.ljpar = static_cast<const LJPar*>(ctx->d_table) + size_t(g) * MAXT;
+      KernelFn fn = isLJ ? (tric ? k_grid_build_q<true, true> : k_grid_build_q<true, false>)
+                         : (tric ? k_grid_build_q<false, true> : k_grid_build_q<false, false>);
+      hipLaunchKernelGGL(fn, dim3(A.ntiles), dim3(256), 0, ctx->stream, A);
+      RG_HIP(ctx, hipGetLastError());
+    }
+    if (sync) RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return RG_OK;
+  }
   // parameter tables of all launches, copied once into the context's persistent table buffer
   struct Launch
   {
@@ -908,8 +1119,11 @@ rg_status gridCheck(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw
   if (probe < 0 || probe >= ff->n) return rg::fail(ctx, RG_ERR_INVALID, "%s: probe pseudo-atom out of range", who);
   rg_framework* fw = const_cast<rg_framework*>(fwc);
   for (const rg_grid* g : {lj, coul})
+  {
     if (g && std::memcmp(g->cell, fw->cell, sizeof(g->cell)) != 0)
       return rg::fail(ctx, RG_ERR_INVALID, "%s: grid cell differs from the framework unit cell", who);
+    if (g && g->values != 8) return rg::fail(ctx, RG_ERR_UNSUPPORTED, "%s: tricubic grids only", who);
+  }
   std::vector<int> typeRow, rowType;
   {
     const rg_status st = frameworkTypeRows(ctx, ff, fw, typeRow, rowType, who);
@@ -1030,8 +1244,16 @@ extern "C"
 
 rg_status rg_grid_create(rg_ctx* ctx, const double* cell9, int is_rectangular, int nx, int ny, int nz, rg_grid** out)
 {
+  return rg_grid_create_scheme(ctx, cell9, is_rectangular, nx, ny, nz, RG_SCHEME_TRICUBIC, out);
+}
+
+rg_status rg_grid_create_scheme(rg_ctx* ctx, const double* cell9, int is_rectangular, int nx, int ny, int nz,
+                                rg_interpolation_scheme scheme, rg_grid** out)
+{
   if (!ctx || !cell9 || !out || nx < 2 || ny < 2 || nz < 2)
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_create: invalid arguments");
+  if (scheme != RG_SCHEME_TRICUBIC && scheme != RG_SCHEME_TRIQUINTIC)
+    return rg::fail(ctx, RG_ERR_UNSUPPORTED, "rg_grid_create: interpolation scheme %d not supported", int(scheme));
   rg_grid* g = new rg_grid();
   g->ctx = ctx;
   g->device = ctx->device;
@@ -1041,7 +1263,8 @@ rg_status rg_grid_create(rg_ctx* ctx, const double* cell9, int is_rectangular, i
   g->n[0] = nx;
   g->n[1] = ny;
   g->n[2] = nz;
-  g->count = 8 * size_t(nx) * size_t(ny) * size_t(nz);
+  g->values = int(scheme);
+  g->count = size_t(g->values) * size_t(nx) * size_t(ny) * size_t(nz);
   hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&g->d, g->count * sizeof(double));
   if (e != hipSuccess)

@@ -65,6 +65,7 @@ struct rg_grid
   double cell[9], inv[9];
   int rect = 1;
   int n[3] = {0, 0, 0};
+  int values = 8;    // per point: 8 (tricubic) or 27 (triquintic)
   size_t count = 0;  // doubles
   double* d = nullptr;
 };

@@ -154,6 +154,87 @@ def test_interpolation_matches_oracle(gpu_ctx):
     assert np.allclose(grad, go, rtol=1e-9, atol=1e-6)
 
 
+def test_interpolation_hessian_matches_oracle(gpu_ctx):
+    """Tricubic energies, gradients and Hessians of the interpolant at 5,000 random positions."""
+    print("case: tricubic hessian", flush=True)
+    cell, rect = w.cell_from_lengths_angles(12.0, 13.0, 12.5, 85.0, 95.0, 100.0)
+    rng = np.random.default_rng(21)
+    dims = (9, 10, 8)
+    data = rng.normal(size=(dims[2], dims[1], dims[0], 8))
+    g = ra.Grid(gpu_ctx, cell, rect, dims)
+    g.upload(data)
+    pos = rng.uniform(-0.5, 1.5, (5000, 3)) @ cell.T
+    e, gr, h = g.interpolate_hessian(pos)
+    eo, go, ho = oracle.interpolate_hessian(cell, rect, dims, data, pos)
+    np.testing.assert_allclose(e, eo, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(gr, go, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(h, ho, rtol=1e-9, atol=1e-9)
+
+
+def _quintic_case(triclinic):
+    if triclinic:
+        cell, rect = w.cell_from_lengths_angles(9.0, 10.0, 11.0, 80.0, 95.0, 105.0)
+    else:
+        cell, rect = np.diag([9.5, 10.5, 11.0]), True
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0, 1, (10, 3)) @ cell.T
+    typ = np.array([0, 1] * 5, np.int32)
+    q = np.array([1.3, -1.3] * 5)
+    return cell, rect, pos, typ, q
+
+
+@pytest.mark.parametrize("triclinic", [False, True])
+def test_triquintic_grids_match_oracle(gpu_ctx, triclinic):
+    """Triquintic LJ and Ewald real-space grids (27 values per point) at every point of a small framework with a
+    supercell, against the oracle's values; per channel the absolute bound also allows 1e-12 of the channel's
+    largest magnitude (the moments of the device builder group the same pair terms differently)."""
+    print(f"case: triquintic grids triclinic={triclinic}", flush=True)
+    cell, rect, pos, typ, q = _quintic_case(triclinic)
+    fw = w.Framework("small", cell, rect, pos, typ, q)
+    spec = w.forcefield_config1()
+    dims = [6, 5, 7]
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    gs = [ra.Grid(gpu_ctx, cell, rect, dims, scheme=27) for _ in range(2)]
+    ra.build_grids(gpu_ctx, ff, f, [(gs[0], RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0),
+                                    (gs[1], RG_GRID_EWALD_REAL, 0, 12.0)])
+    pairs = oracle.pairs_from_spec(spec)
+    ijk = np.array([(i, j, k) for k in range(dims[2]) for j in range(dims[1]) for i in range(dims[0])])
+    for g, gt, pr in ((gs[0], 0, spec.index("O_co2")), (gs[1], 3, 0)):
+        ref = oracle.grid_points_q(cell, rect, dims, gt, pos, typ, q, pairs, pr, 12.0, 12.0, spec.alpha, ijk,
+                                   nthreads=NTHREADS)
+        got = g.download().reshape(ref.shape)
+        bound = ABS + REL * np.abs(ref) + 1e-12 * np.abs(ref).max(axis=0)
+        bad = np.argwhere(np.abs(got - ref) > bound)
+        assert bad.size == 0, f"type {gt}: {len(bad)} values out of tolerance, first {bad[:5]}"
+        # interpolation: energy, gradient and Hessian at random positions against the oracle on the same data
+        rng = np.random.default_rng(3 + gt)
+        pts = rng.uniform(-0.2, 1.2, (3000, 3)) @ cell.T
+        e, gr, h = g.interpolate_hessian(pts)
+        eo, go, ho = oracle.interpolate_q(cell, rect, dims, got.reshape(dims[2], dims[1], dims[0], 27), pts)
+        for a, b, what in ((e, eo, "energy"), (gr, go, "gradient"), (h, ho, "hessian")):
+            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max(), err_msg=what)
+        e2 = g.interpolate(pts)
+        np.testing.assert_allclose(e2, e, rtol=1e-13, atol=1e-13 * np.abs(e).max())
+
+
+def test_triquintic_grid_is_rejected_where_unsupported(gpu_ctx):
+    cell, rect, pos, typ, q = _quintic_case(True)
+    fw = w.Framework("small", cell, rect, pos, typ, q)
+    spec = w.forcefield_config1()
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    g3 = ra.Grid(gpu_ctx, cell, rect, [4, 4, 4])
+    g5 = ra.Grid(gpu_ctx, cell, rect, [4, 4, 4], scheme=27)
+    with pytest.raises(RGError, match="scheme"):
+        ra.build_grids(gpu_ctx, ff, f, [(g3, RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0),
+                                        (g5, RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0)])
+    with pytest.raises(RGError, match="tricubic"):
+        ra.grid_check(gpu_ctx, ff, f, g5, spec.index("O_co2"), n_points=10)
+    with pytest.raises(RGError):
+        ra.Grid(gpu_ctx, cell, rect, [4, 4, 4], scheme=1)
+
+
 def _check_close(g, o):
     assert g["n_points"] == o["n_points"] and g["n_coulomb_points"] == o["n_coulomb_points"]
     for part in ("lennard_jones", "coulomb"):

@@ -158,6 +158,16 @@ def test_tricubic_matrix_reproduces_cubic_polynomials():
     e = oracle.interpolate(cell, True, dims, grid, pts @ cell.T)
     expect = np.array([poly(*p) for p in pts])
     assert np.allclose(e, expect, rtol=0, atol=1e-10)
+    # gradient and Hessian of the interpolant (Cartesian: d/dr_a = d/ds_a / L_a for this diagonal cell)
+    L = np.diag(cell)
+    eh, gh, hh = oracle.interpolate_hessian(cell, True, dims, grid, pts @ cell.T)
+    assert np.allclose(eh, expect, rtol=0, atol=1e-10)
+    for q, p in enumerate(pts):
+        g = np.array([poly(*p, *(int(x == a) for x in range(3))) for a in range(3)]) / L
+        h = np.array([[poly(*p, *(int(x == a) + int(x == b) for x in range(3))) for b in range(3)]
+                      for a in range(3)]) / np.outer(L, L)
+        assert np.allclose(gh[q], g, rtol=1e-10, atol=1e-10)
+        assert np.allclose(hh[q], h, rtol=1e-10, atol=1e-10)
     # node values (s = 1 wraps to 0 through fract(), src/mathkit/double3.cpp:22-45, so interior nodes only)
     nodes = np.array([[1, 2, 3], [0, 0, 0], [1, 1, 1]]) / nc
     en = oracle.interpolate(cell, True, dims, grid, nodes @ cell.T)
