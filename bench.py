@@ -47,6 +47,11 @@ BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 FLOP_LJ_TRIC = 79
 FLOP_EWALD_TRIC = 86
+# FP64 FLOP per in-cutoff pair of the triclinic triquintic kernels, counted as for the tricubic ones (FMA = 2,
+# reciprocal 1, square root, exp and erfc not counted): distance 8, fractional offsets 3, radial factors f0..f6 44
+# (LJ) or 64 (Ewald), monomials 26, 61 moment FMAs 122
+FLOP_LJ_QUINTIC = 203
+FLOP_EWALD_QUINTIC = 223
 GRID_BYTES_PER_SITE = 512      # 8 corner chunks of 64 B per tricubic interpolation
 MLP_FLOP_PER_SAMPLE = 1.08e6   # forward 2 * 180.5k + backward 4 * 180.5k (GEMM work of a training sample)
 # HBM bytes per launch of the Ewald grid kernel on the headline workload, from two rocprofv3 --pmc runs of
@@ -182,7 +187,7 @@ def bench_grids(ctx, args):
                                            frac=round(lj_achieved / FP64_PEAK_TFLOPS, 4),
                                            flop_per_launch=float(flop_lj))),
                 step_achieved=round((flop_q + flop_lj) / ((lj_ms + q_ms) / 1000.0) / 1e12, 3))
-    state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, dims=dims)
+    state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, dims=dims, pairs=(n_all, n_lj))
     # the test the reference runs after building (1e5 random points per LJ grid, with the Ewald grid scaled by the
     # probe's charge); outside the timed region
     check = {}
@@ -198,6 +203,58 @@ def bench_grids(ctx, args):
     state["grid_check"] = check
     log(f"grid check (1e5 points per LJ grid): {json.dumps(check)}")
     return value, ms_per_step, npts, roof, state
+
+
+def bench_grids_quintic(ctx, g1, args):
+    """Triquintic grids (27 values per point) of the headline workload: C_co2 LJ, O_co2 LJ and the unit-charge Ewald
+    real-space grid, one launch each (3 x 46.6M points, 30.2 GB FP64).  Per-kernel HIP-event times over the timed
+    steps; FP64 FLOP/s with the counted FLOP per in-cutoff pair of the triclinic kernels (FLOP_*_QUINTIC)."""
+    import torch
+
+    import raspa3_amd as ra
+    from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES, RG_SCHEME_TRIQUINTIC
+
+    fw, spec, ff, f, dims = g1["fw"], g1["spec"], g1["ff"], g1["f"], g1["dims"]
+    npts = dims[0] * dims[1] * dims[2]
+    grids = [ra.Grid(ctx, fw.cell, fw.rect, dims, scheme=RG_SCHEME_TRIQUINTIC) for _ in range(3)]
+    specs = [(grids[0], RG_GRID_LENNARD_JONES, spec.index("C_co2"), spec.rc_framework),
+             (grids[1], RG_GRID_LENNARD_JONES, spec.index("O_co2"), spec.rc_framework),
+             (grids[2], RG_GRID_EWALD_REAL, 0, spec.rc_coulomb)]
+    for _ in range(max(1, args.warmup)):
+        ra.build_grids(ctx, ff, f, specs)
+    stream = torch.cuda.ExternalStream(ctx.stream, device=torch.device("cuda", 0))
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        for g in range(3):
+            ev[k][g].record(stream)
+            ra.build_grids(ctx, ff, f, specs[g:g + 1], sync=False)
+        ev[k][3].record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1000.0 / args.steps
+    kms = [sum(e[g].elapsed_time(e[g + 1]) for e in ev) / args.steps for g in range(3)]
+    n_all, n_lj = g1["pairs"]
+    flop_lj = npts * FLOP_LJ_QUINTIC * n_lj
+    flop_q = npts * FLOP_EWALD_QUINTIC * n_all
+    out = dict(points_per_s=3 * npts / (ms_per_step / 1000.0), ms_per_step=round(ms_per_step, 3),
+               points_per_step=3 * npts, bytes_written_per_step=3 * npts * 27 * 8,
+               kernels={"lj_C_co2": round(kms[0], 3), "lj_O_co2": round(kms[1], 3), "ewald": round(kms[2], 3)},
+               flop_per_pair=dict(lj=FLOP_LJ_QUINTIC, ewald=FLOP_EWALD_QUINTIC))
+    out["ewald_achieved_tflops"] = round(flop_q / (kms[2] / 1000.0) / 1e12, 3)
+    out["ewald_frac"] = round(out["ewald_achieved_tflops"] / FP64_PEAK_TFLOPS, 4)
+    out["lj_achieved_tflops"] = round(flop_lj / (kms[0] / 1000.0) / 1e12, 3)
+    out["lj_frac"] = round(out["lj_achieved_tflops"] / FP64_PEAK_TFLOPS, 4)
+    log(f"triquintic grids: ms/step {ms_per_step:.3f} (kernels {kms}); {out['points_per_s']:.4e} grid points/s")
+    # an interpolation spot check of the built LJ grid: Hessians at 1e5 random points (outside the timed region)
+    pts = np.random.default_rng(3).uniform(0, 1, (100000, 3)) @ fw.cell.T
+    t = time.perf_counter()
+    e, _, h = grids[0].interpolate_hessian(pts)
+    out["hessian_points_per_s_host_roundtrip"] = round(len(pts) / (time.perf_counter() - t), 1)
+    del grids
+    return out
 
 
 def bench_trial_energies(ctx, g1, args):
@@ -510,6 +567,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--grids-only", action="store_true", help="measure the headline grid build only")
+    ap.add_argument("--quintic-only", action="store_true",
+                    help="measure the headline grid build and the triquintic grids of the same workload only")
     ap.add_argument("--widom-insertions", type=int, default=1 << 20)
     ap.add_argument("--gcmc-systems", type=int, default=256)
     ap.add_argument("--gcmc-equilibration", type=int, default=600)
@@ -550,7 +609,10 @@ def main():
     ctx = ra.Context(0)
     value, ms_per_step, npts, roof, g1 = bench_grids(ctx, args)
     extra = {"grid_check": g1["grid_check"]}
-    if not args.grids_only:
+    if args.quintic_only:
+        extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
+    if not args.grids_only and not args.quintic_only:
+        extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)
         extra["gcmc"] = bench_gcmc(ctx, args)
