@@ -416,7 +416,9 @@ class MC:
             if coul is not None:
                 cg = _f(coul).reshape(-1)
                 self._keep.append(cg)
-            gdims = _n(dims)
+            # the fourth entry is the scheme: values per grid point, 8 (tricubic) or 27 (triquintic)
+            values = next(iter(gmap.values())).shape[-1] if gmap else (coul.shape[-1] if coul is not None else 8)
+            gdims = _n(list(dims) + [values])
             self._keep.append(gdims)
         self.slots = max(int(d["n_atoms"]) for d in comps)
         ci = np.zeros(COMP_INT * len(comps), np.int32)

@@ -1314,5 +1314,5 @@ int orc_select_trial(int n, const double* lbf, double uniform)
 
 }  // extern "C"
 
-#include "oracle_mc.inc"
 #include "oracle_quintic.inc"
+#include "oracle_mc.inc"

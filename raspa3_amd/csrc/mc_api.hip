@@ -163,19 +163,16 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
         v.n[a] = g->n[a];
         v.ncd[a] = double(g->n[a] - 1);
       }
+      v.values = g->values;
       return v;
     };
     for (int g = 0; g < n_grids; ++g)
     {
       if (!grids[g] || grid_types[g] < 0 || grid_types[g] >= ff->n)
         return bail(fail(ctx, RG_ERR_INVALID, "rg_system_create: grid %d invalid", g));
-      if (grids[g]->values != 8)
-        return bail(fail(ctx, RG_ERR_UNSUPPORTED, "rg_system_create: grid %d is not tricubic", g));
       got[grid_types[g]] = static_cast<int>(s->gridViews.size());
       s->gridViews.push_back(view(grids[g]));
     }
-    if (coulomb_grid && coulomb_grid->values != 8)
-      return bail(fail(ctx, RG_ERR_UNSUPPORTED, "rg_system_create: the Coulomb grid is not tricubic"));
     if (coulomb_grid)
     {
       h.coulGrid = static_cast<int>(s->gridViews.size());

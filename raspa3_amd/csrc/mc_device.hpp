@@ -74,7 +74,7 @@ __device__ __forceinline__ double coul0(double alpha, double scaling, double r, 
 }
 
 // ---------------------------------------------------------------------------------------------------
-// Tricubic Hermite evaluation of one cell (see interpolate.hip for the derivation).
+// Tricubic (and triquintic) Hermite evaluation of one cell (see interpolate.hip for the derivation).
 // ---------------------------------------------------------------------------------------------------
 struct GridView
 {
@@ -82,6 +82,7 @@ struct GridView
   double inv[9];
   int n[3];
   double ncd[3];
+  int values;  // 8 (tricubic) or 27 (triquintic)
 };
 
 __device__ __forceinline__ void hermite(double t, double w[2][2])
@@ -91,6 +92,49 @@ __device__ __forceinline__ void hermite(double t, double w[2][2])
   w[0][1] = t3 - 2.0 * t2 + t;
   w[1][0] = -2.0 * t3 + 3.0 * t2;
   w[1][1] = t3 - t2;
+}
+
+// Triquintic energy of one cell: tensor product of 1-D quintic Hermite bases (value, first and second derivative at
+// both ends); value l of a record carries the derivative orders (ox, oy, oz) of the reference's layout
+// (src/raspakit/interpolation_energy_grid.cpp:3886-3924).
+__device__ __forceinline__ void hermite5(double t, double w[2][3])
+{
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+  w[0][0] = 1.0 - 10.0 * t3 + 15.0 * t4 - 6.0 * t5;
+  w[0][1] = t - 6.0 * t3 + 8.0 * t4 - 3.0 * t5;
+  w[0][2] = 0.5 * t2 - 1.5 * t3 + 1.5 * t4 - 0.5 * t5;
+  w[1][0] = 10.0 * t3 - 15.0 * t4 + 6.0 * t5;
+  w[1][1] = -4.0 * t3 + 7.0 * t4 - 3.0 * t5;
+  w[1][2] = 0.5 * t3 - t4 + 0.5 * t5;
+}
+
+__device__ __forceinline__ double interpolateQuintic(const GridView& g, const size_t c0[3], const double t[3])
+{
+  constexpr int kOrd[27][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {1, 1, 0}, {1, 0, 1},
+                               {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1},
+                               {0, 2, 1}, {1, 0, 2}, {0, 1, 2}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {2, 1, 1},
+                               {1, 2, 1}, {1, 1, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+  double wx[2][3], wy[2][3], wz[2][3];
+  hermite5(t[0], wx);
+  hermite5(t[1], wy);
+  hermite5(t[2], wz);
+  double e = 0.0;
+  for (int c = 0; c < 8; ++c)
+  {
+    const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+    const double* F = g.data + 27 * ((c0[0] + cx) + size_t(g.n[0]) * ((c0[1] + cy) + size_t(g.n[1]) * (c0[2] + cz)));
+    double X[3], Y[3], Z[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+      X[d] = cx ? wx[1][d] : wx[0][d];
+      Y[d] = cy ? wy[1][d] : wy[0][d];
+      Z[d] = cz ? wz[1][d] : wz[0][d];
+    }
+#pragma unroll
+    for (int l = 0; l < 27; ++l) e = fma(F[l], X[kOrd[l][0]] * Y[kOrd[l][1]] * Z[kOrd[l][2]], e);
+  }
+  return e;
 }
 
 __device__ __forceinline__ double interpolate(const GridView& g, double px, double py, double pz)
@@ -110,6 +154,7 @@ __device__ __forceinline__ double interpolate(const GridView& g, double px, doub
     c0[a] = c;
     t[a] = v * g.ncd[a] - double(c);
   }
+  if (g.values == 27) return interpolateQuintic(g, c0, t);
   double wx[2][2], wy[2][2], wz[2][2];
   hermite(t[0], wx);
   hermite(t[1], wy);

@@ -88,18 +88,18 @@ def small_charged_setup(ctx, spec, types, spacing=0.3):
     return fw, ff, f, dims, grids, coul, host, hcoul
 
 
-def itq29_setup(ctx, spec, types, spacing=0.3):
-    """Charged ITQ-29, 3 x 3 x 3 cells (35.6 A), grids built on the unit cell."""
+def itq29_setup(ctx, spec, types, spacing=0.3, scheme=8):
+    """Charged ITQ-29, 3 x 3 x 3 cells (35.6 A), grids built on the unit cell (tricubic or triquintic)."""
     fw, unit = w.framework_itq29(spec, 3)
     ff = ra.ForceField(ctx, spec)
     fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw.pos[:72], fw.type[:72], fw.charge[:72]))
     dims = ra.grid_dims(unit, spacing)
-    grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
-    coul = ra.Grid(ctx, unit, True, dims)
+    grids = {t: ra.Grid(ctx, unit, True, dims, scheme=scheme) for t in types}
+    coul = ra.Grid(ctx, unit, True, dims, scheme=scheme)
     ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
                    [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
-    host = {t: g.download().reshape(dims[2], dims[1], dims[0], 8) for t, g in grids.items()}
-    hcoul = coul.download().reshape(dims[2], dims[1], dims[0], 8)
+    host = {t: g.download().reshape(dims[2], dims[1], dims[0], scheme) for t, g in grids.items()}
+    hcoul = coul.download().reshape(dims[2], dims[1], dims[0], scheme)
     return fw, unit, ff, ra.Framework(ctx, fw), dims, grids, coul, host, hcoul
 
 
@@ -121,12 +121,14 @@ def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, 
     return g
 
 
-def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx):
-    """Two independent CO2 systems in charged ITQ-29 (grid path, Ewald kmax 6) for 5 + 15 cycles at 1e5 Pa."""
-    print("case: gcmc CO2 replay", flush=True)
+@pytest.mark.parametrize("scheme", [8, 27])
+def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx, scheme):
+    """Two independent CO2 systems in charged ITQ-29 (grid path, tricubic or triquintic grids, Ewald kmax 6) for
+    5 + 15 cycles at 1e5 Pa."""
+    print(f"case: gcmc CO2 replay, {scheme} values per grid point", flush=True)
     spec = w.forcefield_zeolite()
     types = [spec.index("C_co2"), spec.index("O_co2")]
-    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types, scheme=scheme)
     comp = w.co2_reference(spec)
     params = w.mc_params(pressure_pa=1e5, seed=5, max_molecules=128, production_cycles=15)
     systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
