@@ -171,20 +171,24 @@ rg_status rg_grid_interpolate_hessian(rg_ctx* ctx, const rg_grid* grid, int n, c
                                       double* gradient, double* hessian);
 
 /* Test of a built grid on random points, as System::createFrameworkInterpolationGrids does after building
- * (src/raspakit/system_interpolation_grids.cpp:240-840, numberOfGridTestPoints = 1e5 by default): the tricubic
+ * (src/raspakit/system_interpolation_grids.cpp:240-840, numberOfGridTestPoints = 1e5 by default): the
  * interpolation of the Lennard-Jones grid of pseudo-atom 'probe' (and, when coulomb_grid is non-null, of the
  * unit-charge Ewald real-space grid scaled by probe_charge) against the analytic sums of
- * Interactions::calculateTricubicFractionalAtPosition at the same points.  Points are uniform in the unit cell:
+ * Interactions::calculateTricubicFractionalAtPosition / calculateTriquinticFractionalAtPosition at the same points
+ * (both grids of one scheme; Hessians for the triquintic scheme).  Points are uniform in the unit cell:
  * frac (n_points * 3 fractional coordinates) when non-null, else drawn from a Philox4x32-10 stream keyed by seed.
  * Every point is weighted by exp(-beta E_LJ) with beta = 1/(k_B T) in inverse internal energy units and E_LJ in K,
  * as the reference computes it; the Ewald statistics take the points whose E_LJ (K) is below the overlap criterion.
- * Energies in K, gradients in K/A (Cartesian). */
+ * Energies in K, gradients in K/A, Hessians in K/A^2 (Cartesian). */
 typedef struct rg_grid_check_stats
 {
   double weight;                      /* sum of the Boltzmann weights */
   double energy_table, energy_full;   /* Boltzmann averages, interpolated and analytic */
   double energy_relative_error;       /* sqrt(sum w (full - table)^2 / sum w full^2) */
   double gradient_table[3], gradient_full[3], gradient_relative_error[3];
+  /* Cartesian Hessian components xx, xy, xz, yy, yz, zz in K/A^2: triquintic grids only, zero for tricubic ones
+   * (the reference compares Hessians for the triquintic scheme only, system_interpolation_grids.cpp:694-744) */
+  double hessian_table[6], hessian_full[6], hessian_relative_error[6];
 } rg_grid_check_stats;
 
 typedef struct rg_grid_check_t

@@ -51,7 +51,7 @@ def lib(native=False):
                                                C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]
         L.orc_grid_check.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, ip, dp, C.c_int, dp, dp,
                                      dp, ip, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp,
-                                     C.c_double, C.c_double, C.c_int, dp, dp]
+                                     C.c_double, C.c_double, C.c_int, dp, C.c_int, dp]
         L.orc_framework_energy.argtypes = [dp, C.c_int, C.c_int, dp, ip, dp, dp, dp, C.c_int, dp, dp, dp, ip,
                                            C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, dp, ip,
                                            dp, dp, dp, C.c_int, dp, C.c_int, ip, ip, C.POINTER(dp), dp, dp]
@@ -226,23 +226,27 @@ def interpolate(cell, rect, dims, grid, pos, gradient=False):
 
 def _check_stats(o):
     return dict(weight=o[0], energy_table=o[1], energy_full=o[2], energy_relative_error=o[3],
-                gradient_table=o[4:7].copy(), gradient_full=o[7:10].copy(), gradient_relative_error=o[10:13].copy())
+                gradient_table=o[4:7].copy(), gradient_full=o[7:10].copy(), gradient_relative_error=o[10:13].copy(),
+                hessian_table=o[13:19].copy(), hessian_full=o[19:25].copy(), hessian_relative_error=o[25:31].copy())
 
 
 def grid_check(cell, rect, dims, fw_pos, fw_type, fw_q, pairs, probe, rc_vdw, rc_supercell, alpha, lj_grid, frac,
                temperature, coulomb_grid=None, probe_charge=0.0, overlap=1e6):
-    """The reference's grid test (system_interpolation_grids.cpp:240-840, tricubic) at the given fractional points:
-    dict(n_points, lennard_jones, n_coulomb_points, coulomb) with Boltzmann-weighted averages and relative errors."""
+    """The reference's grid test (system_interpolation_grids.cpp:240-840) at the given fractional points, for
+    tricubic ((..., 8) arrays) or triquintic ((..., 27) arrays, with Hessians) grids: dict(n_points, lennard_jones,
+    n_coulomb_points, coulomb) with Boltzmann-weighted averages and relative errors."""
     c9, r, na, p, t, q, nt, e, s, sh, lj = _fw_args(cell, rect, fw_pos, fw_type, fw_q, pairs)
     frac = _f(frac).reshape(-1, 3)
     g = _f(lj_grid).reshape(-1)
+    values = g.size // (dims[0] * dims[1] * dims[2])  # 8 or 27 values per point
+    assert values in (8, 27) and values * dims[0] * dims[1] * dims[2] == g.size
     cg = None if coulomb_grid is None else _f(coulomb_grid).reshape(-1)
-    out = np.zeros(28)
+    out = np.zeros(64)
     lib().orc_grid_check(c9, r, *dims, na, p, t, q, nt, e, s, sh, lj, probe, rc_vdw, rc_supercell, alpha, overlap,
                          _d(g), _d(cg) if cg is not None else None, probe_charge, temperature, len(frac), _d(frac),
-                         _d(out))
-    return dict(n_points=int(out[0]), lennard_jones=_check_stats(out[1:14]), n_coulomb_points=int(out[14]),
-                coulomb=_check_stats(out[15:28]))
+                         int(values), _d(out))
+    return dict(n_points=int(out[0]), lennard_jones=_check_stats(out[1:32]), n_coulomb_points=int(out[32]),
+                coulomb=_check_stats(out[33:64]))
 
 
 def tricubic_matrix():

@@ -520,6 +520,8 @@ struct Site
 
 }  // namespace
 
+#include "oracle_quintic.inc"
+
 // ===================================================================================================
 // C entry points (ctypes)
 // ===================================================================================================
@@ -851,13 +853,16 @@ void orc_interpolate_hessian(const double* cell9, int rect, int nx, int ny, int 
 // calculateTricubicFractionalAtPosition over the supercell, its fractional gradient turned Cartesian with the
 // transposed inverse cell (:302-305); weights exp(-beta E_LJ) with beta in inverse internal units and E_LJ in K
 // (:365, :545); the Ewald part over the points with E_LJ (K) below the overlap criterion (:420).
-// out[28] = {n, LJ stats[13], n_coulomb, Coulomb stats[13]}; stats = {sum w, <E> table, <E> full, relative error,
-// <grad> table[3], <grad> full[3], relative error[3]}, energies in K, gradients in K/A.
+// out[64] = {n, LJ stats[31], n_coulomb, Coulomb stats[31]}; stats = {sum w, <E> table, <E> full, relative error,
+// <grad> table[3], <grad> full[3], relative error[3], <hessian> table[6], <hessian> full[6], relative error[6]},
+// energies in K, gradients in K/A, Hessians (xx xy xz yy yz zz) in K/A^2.  values = 8 (tricubic; Hessian entries
+// zero) or 27 (triquintic: calculateTriquinticFractionalAtPosition and interpolateHessian,
+// system_interpolation_grids.cpp:328-441).
 void orc_grid_check(const double* cell9, int rect, int nx, int ny, int nz, int natoms, const double* pos,
                     const int* type, const double* charge, int ntypes, const double* pair_eps, const double* pair_sigma,
                     const double* pair_shift, const int* pair_lj, int probe, double rc_vdw, double rc_supercell,
                     double alpha, double overlap, const double* lj_grid, const double* coul_grid, double probe_charge,
-                    double temperature, int n, const double* frac, double* out)
+                    double temperature, int n, const double* frac, int values, double* out)
 {
   Box unit = makeBox(cell9, rect);
   double w[3];
@@ -882,7 +887,67 @@ void orc_grid_check(const double* cell9, int rect, int nx, int ny, int nz, int n
   {
     double w = 0, t = 0, f = 0, d2 = 0, f2 = 0, gt[3] = {0, 0, 0}, gf[3] = {0, 0, 0}, gd2[3] = {0, 0, 0},
            gf2[3] = {0, 0, 0};
+    double ht[6] = {}, hf[6] = {}, hd2[6] = {}, hf2[6] = {};
   } sl, sc;
+  const bool quintic = values == 27;
+  auto addH = [](Sums& S, double wt, const double* ht, const double* hf)
+  {
+    for (int a = 0; a < 6; ++a)
+    {
+      S.ht[a] += ht[a] * wt;
+      S.hf[a] += hf[a] * wt;
+      S.hd2[a] += (hf[a] - ht[a]) * (hf[a] - ht[a]) * wt;
+      S.hf2[a] += hf[a] * hf[a] * wt;
+    }
+  };
+  // interpolated and analytic values of one grid type at p: energy, Cartesian gradient, Hessian (6 components)
+  auto eval = [&](int gridType, const double* grid, const double p[3], double scale, double& et, double gt[3],
+                  double htab[6], double& ef, double gf[3], double hfull[6])
+  {
+    if (!quintic)
+    {
+      double v[8];
+      interpolateOne(unit, np, grid, p, &et, gt);
+      tricubicFractionalAtPosition(gridType, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
+                                   sq.data(), v);
+      ef = v[0];
+      for (int c = 0; c < 3; ++c) gf[c] = unit.inv.c[c][0] * v[1] + unit.inv.c[c][1] * v[2] + unit.inv.c[c][2] * v[3];
+      for (int a = 0; a < 6; ++a) htab[a] = hfull[a] = 0.0;
+    }
+    else
+    {
+      double v[27], h9[9];
+      interpolateQuinticOne(unit, np, grid, p, &et, gt, h9);
+      quinticFractionalAtPosition(gridType, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
+                                  sq.data(), v);
+      ef = v[0];
+      for (int c = 0; c < 3; ++c) gf[c] = unit.inv.c[c][0] * v[1] + unit.inv.c[c][1] * v[2] + unit.inv.c[c][2] * v[3];
+      // inverseCell^T H inverseCell with H the fractional second derivatives (values 4-9)
+      const double Hs[3][3] = {{v[4], v[5], v[6]}, {v[5], v[7], v[8]}, {v[6], v[8], v[9]}};
+      double Hc[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+          double acc = 0.0;
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) acc += unit.inv.c[i][a] * Hs[a][b] * unit.inv.c[j][b];
+          Hc[i][j] = acc;
+        }
+      const int ii[6] = {0, 0, 0, 1, 1, 2}, jj[6] = {0, 1, 2, 1, 2, 2};
+      for (int a = 0; a < 6; ++a)
+      {
+        htab[a] = h9[3 * ii[a] + jj[a]] * scale;
+        hfull[a] = Hc[ii[a]][jj[a]] * scale;
+      }
+    }
+    et *= scale;
+    ef *= scale;
+    for (int c = 0; c < 3; ++c)
+    {
+      gt[c] *= scale;
+      gf[c] *= scale;
+    }
+  };
   auto add = [](Sums& S, double wt, double et, double ef, const double* gt, const double* gf)
   {
     S.w += wt;
@@ -898,37 +963,22 @@ void orc_grid_check(const double* cell9, int rect, int nx, int ny, int nz, int n
       S.gf2[a] += gf[a] * gf[a] * wt;
     }
   };
-  // Cartesian gradient from fractional derivatives: inverseCell^T (v1, v2, v3)
-  auto cart = [&](const double v[8], double scale, double g[3])
-  {
-    for (int c = 0; c < 3; ++c)
-      g[c] = scale * (unit.inv.c[c][0] * v[1] + unit.inv.c[c][1] * v[2] + unit.inv.c[c][2] * v[3]);
-  };
   long long nq = 0;
   for (int i = 0; i < n; ++i)
   {
     double p[3];
     mulv(unit.cell, frac + 3 * i, p);
-    double et, gt[3], v[8], gf[3];
-    interpolateOne(unit, np, lj_grid, p, &et, gt);
-    et *= K;
-    for (double& x : gt) x *= K;
-    tricubicFractionalAtPosition(0, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
-                                 sq.data(), v);
-    const double ef = v[0] * K;
-    cart(v, K, gf);
+    double et, gt[3], ht[6], ef, gf[3], hf[6];
+    eval(0, lj_grid, p, K, et, gt, ht, ef, gf, hf);
     const double wt = std::exp(-beta * ef);
     add(sl, wt, et, ef, gt, gf);
+    addH(sl, wt, ht, hf);
     if (coul_grid && ef < overlap)
     {
-      double ct, cgt[3], cv[8], cgf[3];
-      interpolateOne(unit, np, coul_grid, p, &ct, cgt);
-      ct *= probe_charge * K;
-      for (double& x : cgt) x *= probe_charge * K;
-      tricubicFractionalAtPosition(1, ff, rc_vdw, alpha, super, p, probe, unit, ns, spos.data(), stype.data(),
-                                   sq.data(), cv);
-      cart(cv, probe_charge * K, cgf);
-      add(sc, wt, ct, cv[0] * probe_charge * K, cgt, cgf);
+      double ct, cgt[3], cht[6], cf, cgf[3], chf[6];
+      eval(1, coul_grid, p, probe_charge * K, ct, cgt, cht, cf, cgf, chf);
+      add(sc, wt, ct, cf, cgt, cgf);
+      addH(sc, wt, cht, chf);
       ++nq;
     }
   }
@@ -944,12 +994,18 @@ void orc_grid_check(const double* cell9, int rect, int nx, int ny, int nz, int n
       o[7 + a] = S.gf[a] / S.w;
       o[10 + a] = std::sqrt(S.gd2[a] / S.gf2[a]);
     }
+    for (int a = 0; a < 6; ++a)
+    {
+      o[13 + a] = S.ht[a] / S.w;
+      o[19 + a] = S.hf[a] / S.w;
+      o[25 + a] = S.hf2[a] > 0.0 ? std::sqrt(S.hd2[a] / S.hf2[a]) : 0.0;
+    }
   };
-  for (int k = 0; k < 28; ++k) out[k] = 0.0;
+  for (int k = 0; k < 64; ++k) out[k] = 0.0;
   out[0] = n;
   store(sl, out + 1);
-  out[14] = double(nq);
-  if (nq > 0) store(sc, out + 15);
+  out[32] = double(nq);
+  if (nq > 0) store(sc, out + 33);
 }
 
 // The 64x64 tricubic coefficient matrix (row-major), for tests.
@@ -1314,5 +1370,4 @@ int orc_select_trial(int n, const double* lbf, double uniform)
 
 }  // extern "C"
 
-#include "oracle_quintic.inc"
 #include "oracle_mc.inc"

@@ -303,7 +303,7 @@ def build_grids(ctx, ff, fw, specs, sync=True):
 def grid_check(ctx, ff, fw, lj_grid, probe, coulomb_grid=None, probe_charge=0.0, temperature=300.0,
                n_points=100000, frac=None, seed=1):
     """rg_grid_check: the reference's test of a built grid on random points (Boltzmann-weighted averages of the
-    interpolated and analytic energy and gradient, relative errors).  Returns dict(n_points, lennard_jones,
+    interpolated and analytic energy, gradient and, for triquintic grids, Hessian; relative errors).  Returns dict(n_points, lennard_jones,
     n_coulomb_points, coulomb)."""
     out = L.GridCheck()
     f = None if frac is None else L.f64(frac).reshape(-1, 3)
@@ -317,7 +317,9 @@ def grid_check(ctx, ff, fw, lj_grid, probe, coulomb_grid=None, probe_charge=0.0,
         return dict(weight=x.weight, energy_table=x.energy_table, energy_full=x.energy_full,
                     energy_relative_error=x.energy_relative_error, gradient_table=np.array(list(x.gradient_table)),
                     gradient_full=np.array(list(x.gradient_full)),
-                    gradient_relative_error=np.array(list(x.gradient_relative_error)))
+                    gradient_relative_error=np.array(list(x.gradient_relative_error)),
+                    hessian_table=np.array(list(x.hessian_table)), hessian_full=np.array(list(x.hessian_full)),
+                    hessian_relative_error=np.array(list(x.hessian_relative_error)))
 
     return dict(n_points=out.n_points, lennard_jones=stats(out.lennard_jones), n_coulomb_points=out.n_coulomb_points,
                 coulomb=stats(out.coulomb))

@@ -132,7 +132,9 @@ class McState(C.Structure):
 class GridCheckStats(C.Structure):
     _fields_ = [("weight", C.c_double), ("energy_table", C.c_double), ("energy_full", C.c_double),
                 ("energy_relative_error", C.c_double), ("gradient_table", C.c_double * 3),
-                ("gradient_full", C.c_double * 3), ("gradient_relative_error", C.c_double * 3)]
+                ("gradient_full", C.c_double * 3), ("gradient_relative_error", C.c_double * 3),
+                ("hessian_table", C.c_double * 6), ("hessian_full", C.c_double * 6),
+                ("hessian_relative_error", C.c_double * 6)]
 
 
 class GridCheck(C.Structure):

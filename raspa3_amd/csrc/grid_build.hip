@@ -1011,7 +1011,7 @@ struct CheckArgs
   int n;
   int coul;
   double rc2, alpha, alpha2, k2a, coulomb;
-  double* out;  // n * 8: E_lj, dE_lj/dr (3), E_q, dE_q/dr (3) for a unit probe charge; internal units
+  double* out;  // n * 20: E_lj, dE_lj/dr (3), d2E_lj/dr2 (xx xy xz yy yz zz), then the same for a unit probe charge
 };
 
 __global__ __launch_bounds__(256) void k_grid_check(CheckArgs A)
@@ -1022,7 +1022,21 @@ __global__ __launch_bounds__(256) void k_grid_check(CheckArgs A)
   const int i = blockIdx.x * 256 + threadIdx.x;
   const bool live = i < A.n;
   const double px = live ? A.pos[3 * i] : 0.0, py = live ? A.pos[3 * i + 1] : 0.0, pz = live ? A.pos[3 * i + 2] : 0.0;
-  double e = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, eq = 0.0, h0 = 0.0, h1 = 0.0, h2 = 0.0;
+  // per grid type: energy, gradient, Hessian (xx xy xz yy yz zz); Cartesian second derivatives f2 dr_a dr_b + f1 d_ab
+  double v[2][10] = {};
+  auto add = [&](double* o, double e, double f1, double f2, double dx, double dy, double dz)
+  {
+    o[0] += e;
+    o[1] = fma(f1, dx, o[1]);
+    o[2] = fma(f1, dy, o[2]);
+    o[3] = fma(f1, dz, o[3]);
+    o[4] += fma(f2, dx * dx, f1);
+    o[5] = fma(f2, dx * dy, o[5]);
+    o[6] = fma(f2, dx * dz, o[6]);
+    o[7] += fma(f2, dy * dy, f1);
+    o[8] = fma(f2, dy * dz, o[8]);
+    o[9] += fma(f2, dz * dz, f1);
+  };
   for (int j = 0; j < A.n_images; ++j)  // uniform over the wavefront: every lane reads the same image
   {
     const Img& b = A.img[j];
@@ -1034,44 +1048,39 @@ __global__ __launch_bounds__(256) void k_grid_check(CheckArgs A)
       const LJPar& P = A.par[b.type];
       if (P.e4 != 0.0)
       {
-        // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46 (energy and f1)
+        // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46 (energy, f1, f2)
         const double ir2 = rcp64(rr);
         const double x = P.s2 * ir2;
         const double t3 = x * x * x;
         const double t6 = t3 * t3;
-        e += fma(P.e4, t6 - t3, -P.sh);
         const double f1 = P.c1 * fma(2.0, t6, -t3) * ir2;
-        g0 = fma(f1, dx, g0);
-        g1 = fma(f1, dy, g1);
-        g2 = fma(f1, dz, g2);
+        const double f2 = P.c2 * fma(7.0, t6, -2.0 * t3) * ir2 * ir2;
+        add(v[0], fma(P.e4, t6 - t3, -P.sh), f1, f2, dx, dy, dz);
       }
       if (A.coul && b.q != 0.0)
       {
-        // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 (energy and f1)
+        // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 (energy, f1, f2)
         const double ir = rsq64(rr);
         const double r = rr * ir;
         const double q = b.q * A.coulomb;
-        const double qex = q * expNeg(A.alpha2 * rr);
+        const double ar2 = A.alpha2 * rr;
+        const double qex = q * expNeg(ar2);
         const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);
         const double etr = A.k2a * r * qex;
-        eq += ec * ir;
-        const double f1 = -(etr + ec) * ir * ir * ir;
-        h0 = fma(f1, dx, h0);
-        h1 = fma(f1, dy, h1);
-        h2 = fma(f1, dz, h2);
+        const double ir3 = ir * ir * ir;
+        const double f1 = -(etr + ec) * ir3;
+        const double f2 = fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir3 * ir * ir;
+        add(v[1], ec * ir, f1, f2, dx, dy, dz);
       }
     }
   }
   if (!live) return;
-  double* o = A.out + 8 * size_t(i);
-  o[0] = e;
-  o[1] = g0;
-  o[2] = g1;
-  o[3] = g2;
-  o[4] = eq;
-  o[5] = h0;
-  o[6] = h1;
-  o[7] = h2;
+  double* o = A.out + 20 * size_t(i);
+  for (int k = 0; k < 10; ++k)
+  {
+    o[k] = v[0][k];
+    o[10 + k] = v[1][k];
+  }
 }
 
 // Boltzmann-weighted comparison of 'full' (analytic) and 'table' (interpolated) values
@@ -1079,6 +1088,20 @@ struct CheckSums
 {
   double w = 0, table = 0, full = 0, diff2 = 0, full2 = 0;
   double gt[3] = {0, 0, 0}, gf[3] = {0, 0, 0}, gd2[3] = {0, 0, 0}, gf2[3] = {0, 0, 0};
+  double ht[6] = {}, hf[6] = {}, hd2[6] = {}, hf2[6] = {};
+  bool hess = false;
+  // Hessians as 6 components xx xy xz yy yz zz (system_interpolation_grids.cpp:404-441)
+  void addHessian(double wt, const double* htab, const double* hfull)
+  {
+    hess = true;
+    for (int a = 0; a < 6; ++a)
+    {
+      ht[a] += htab[a] * wt;
+      hf[a] += hfull[a] * wt;
+      hd2[a] += (hfull[a] - htab[a]) * (hfull[a] - htab[a]) * wt;
+      hf2[a] += hfull[a] * hfull[a] * wt;
+    }
+  }
   void add(double wt, double et, double ef, const double* gtab, const double* gfull)
   {
     w += wt;
@@ -1106,6 +1129,13 @@ struct CheckSums
       s->gradient_full[a] = gf[a] / w;
       s->gradient_relative_error[a] = std::sqrt(gd2[a] / gf2[a]);
     }
+    if (hess)
+      for (int a = 0; a < 6; ++a)
+      {
+        s->hessian_table[a] = ht[a] / w;
+        s->hessian_full[a] = hf[a] / w;
+        s->hessian_relative_error[a] = std::sqrt(hd2[a] / hf2[a]);
+      }
   }
 };
 
@@ -1122,8 +1152,10 @@ rg_status gridCheck(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw
   {
     if (g && std::memcmp(g->cell, fw->cell, sizeof(g->cell)) != 0)
       return rg::fail(ctx, RG_ERR_INVALID, "%s: grid cell differs from the framework unit cell", who);
-    if (g && g->values != 8) return rg::fail(ctx, RG_ERR_UNSUPPORTED, "%s: tricubic grids only", who);
   }
+  if (coul && coul->values != lj->values)
+    return rg::fail(ctx, RG_ERR_INVALID, "%s: the LJ and Coulomb grids must share the interpolation scheme", who);
+  const bool quintic = lj->values == 27;
   std::vector<int> typeRow, rowType;
   {
     const rg_status st = frameworkTypeRows(ctx, ff, fw, typeRow, rowType, who);
@@ -1152,16 +1184,22 @@ rg_status gridCheck(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw
     for (int c = 0; c < 3; ++c)
       pos[3 * size_t(i) + c] = fw->cell[c] * s[0] + fw->cell[3 + c] * s[1] + fw->cell[6 + c] * s[2];
   }
-  std::vector<double> eLJ(n), gLJ(3 * size_t(n)), eQ, gQ, full(8 * size_t(n));
+  std::vector<double> eLJ(n), gLJ(3 * size_t(n)), hLJ, eQ, gQ, hQ, full(20 * size_t(n));
+  auto interp = [&](const rg_grid* g, std::vector<double>& e, std::vector<double>& gr, std::vector<double>& h)
   {
-    const rg_status st = rg_grid_interpolate(ctx, lj, n, pos.data(), eLJ.data(), gLJ.data());
+    e.resize(n);
+    gr.resize(3 * size_t(n));
+    if (!quintic) return rg_grid_interpolate(ctx, g, n, pos.data(), e.data(), gr.data());
+    h.resize(9 * size_t(n));
+    return rg_grid_interpolate_hessian(ctx, g, n, pos.data(), e.data(), gr.data(), h.data());
+  };
+  {
+    const rg_status st = interp(lj, eLJ, gLJ, hLJ);
     if (st != RG_OK) return st;
   }
   if (coul)
   {
-    eQ.resize(n);
-    gQ.resize(3 * size_t(n));
-    const rg_status st = rg_grid_interpolate(ctx, coul, n, pos.data(), eQ.data(), gQ.data());
+    const rg_status st = interp(coul, eQ, gQ, hQ);
     if (st != RG_OK) return st;
   }
   std::vector<LJPar> par(MAXT);
@@ -1213,20 +1251,44 @@ rg_status gridCheck(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw
   const double K = U.energyToKelvin;
   CheckSums sLJ, sQ;
   long long nQ = 0;
+  // the 6 independent components of an interpolated (row-major 3x3) Hessian, times a scale
+  auto six = [](const double* h9, double sc, double* h6)
+  {
+    h6[0] = h9[0] * sc;
+    h6[1] = h9[1] * sc;
+    h6[2] = h9[2] * sc;
+    h6[3] = h9[4] * sc;
+    h6[4] = h9[5] * sc;
+    h6[5] = h9[8] * sc;
+  };
   for (int i = 0; i < n; ++i)
   {
-    const double* f = &full[8 * size_t(i)];
+    const double* f = &full[20 * size_t(i)];
     const double ef = f[0] * K, et = eLJ[i] * K;
     const double gf[3] = {f[1] * K, f[2] * K, f[3] * K};
     const double gt[3] = {gLJ[3 * size_t(i)] * K, gLJ[3 * size_t(i) + 1] * K, gLJ[3 * size_t(i) + 2] * K};
     const double w = std::exp(-beta * ef);
     sLJ.add(w, et, ef, gt, gf);
+    if (quintic)
+    {
+      double ht[6], hf[6];
+      six(&hLJ[9 * size_t(i)], K, ht);
+      for (int a = 0; a < 6; ++a) hf[a] = f[4 + a] * K;
+      sLJ.addHessian(w, ht, hf);
+    }
     if (coul && ef < ff->overlap)
     {
       const double qK = probeCharge * K;
-      const double qf[3] = {f[5] * qK, f[6] * qK, f[7] * qK};
+      const double qf[3] = {f[11] * qK, f[12] * qK, f[13] * qK};
       const double qt[3] = {gQ[3 * size_t(i)] * qK, gQ[3 * size_t(i) + 1] * qK, gQ[3 * size_t(i) + 2] * qK};
-      sQ.add(w, eQ[i] * qK, f[4] * qK, qt, qf);
+      sQ.add(w, eQ[i] * qK, f[10] * qK, qt, qf);
+      if (quintic)
+      {
+        double ht[6], hf[6];
+        six(&hQ[9 * size_t(i)], qK, ht);
+        for (int a = 0; a < 6; ++a) hf[a] = f[14 + a] * qK;
+        sQ.addHessian(w, ht, hf);
+      }
       ++nQ;
     }
   }

@@ -218,7 +218,7 @@ def test_triquintic_grids_match_oracle(gpu_ctx, triclinic):
         np.testing.assert_allclose(e2, e, rtol=1e-13, atol=1e-13 * np.abs(e).max())
 
 
-def test_triquintic_grid_is_rejected_where_unsupported(gpu_ctx):
+def test_mixed_schemes_are_rejected(gpu_ctx):
     cell, rect, pos, typ, q = _quintic_case(True)
     fw = w.Framework("small", cell, rect, pos, typ, q)
     spec = w.forcefield_config1()
@@ -229,8 +229,8 @@ def test_triquintic_grid_is_rejected_where_unsupported(gpu_ctx):
     with pytest.raises(RGError, match="scheme"):
         ra.build_grids(gpu_ctx, ff, f, [(g3, RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0),
                                         (g5, RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0)])
-    with pytest.raises(RGError, match="tricubic"):
-        ra.grid_check(gpu_ctx, ff, f, g5, spec.index("O_co2"), n_points=10)
+    with pytest.raises(RGError, match="scheme"):
+        ra.grid_check(gpu_ctx, ff, f, g5, spec.index("O_co2"), coulomb_grid=g3, n_points=10)
     with pytest.raises(RGError):
         ra.Grid(gpu_ctx, cell, rect, [4, 4, 4], scheme=1)
 
@@ -245,14 +245,20 @@ def _check_close(g, o):
             assert np.allclose(a[key], b[key], rtol=1e-8, atol=1e-8), (part, key, a[key], b[key])
         assert abs(a["energy_relative_error"] - b["energy_relative_error"]) <= 1e-6 * b["energy_relative_error"]
         assert np.allclose(a["gradient_relative_error"], b["gradient_relative_error"], rtol=1e-6, atol=0)
+        for key in ("hessian_table", "hessian_full"):
+            assert np.allclose(a[key], b[key], rtol=1e-8, atol=1e-8 * max(1.0, np.abs(b[key]).max())), \
+                (part, key, a[key], b[key])
+        assert np.allclose(a["hessian_relative_error"], b["hessian_relative_error"], rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("scheme", [8, 27])
 @pytest.mark.parametrize("case", ["itq29", "triclinic"])
-def test_grid_check_matches_oracle(gpu_ctx, case):
+def test_grid_check_matches_oracle(gpu_ctx, case, scheme):
     """rg_grid_check (the reference's test of a built grid, system_interpolation_grids.cpp:240-840) against the
     oracle's restatement on the same grids and 2000 given points: Boltzmann weights, averages of the interpolated
-    and analytic energies and gradients, relative errors, for the LJ grid and the charge-scaled Ewald grid."""
-    print(f"case: grid check {case}", flush=True)
+    and analytic energies, gradients and (triquintic) Hessians, relative errors, for the LJ grid and the
+    charge-scaled Ewald grid."""
+    print(f"case: grid check {case}, {scheme} values per point", flush=True)
     if case == "itq29":
         spec = w.forcefield_zeolite()
         fw3, cell = w.framework_itq29(spec, 3)
@@ -270,7 +276,7 @@ def test_grid_check_matches_oracle(gpu_ctx, case):
     fw = w.Framework(case, cell, rect, pos, typ, q)
     ff = ra.ForceField(gpu_ctx, spec)
     f = ra.Framework(gpu_ctx, fw)
-    lj, coul = ra.Grid(gpu_ctx, cell, rect, dims), ra.Grid(gpu_ctx, cell, rect, dims)
+    lj, coul = ra.Grid(gpu_ctx, cell, rect, dims, scheme), ra.Grid(gpu_ctx, cell, rect, dims, scheme)
     ra.build_grids(gpu_ctx, ff, f, [(lj, RG_GRID_LENNARD_JONES, probe, 12.0), (coul, RG_GRID_EWALD_REAL, 0, 12.0)])
     frac = np.random.default_rng(21).uniform(0, 1, (2000, 3))
     qp = float(spec.charge[probe])

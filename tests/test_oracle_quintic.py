@@ -200,3 +200,52 @@ def test_interpolation_error_falls_as_h6():
         e, _, _ = oracle.interpolate_q(cell, rect, dims, vals.reshape(n, n, n, 27), pts[sel])
         errs.append(np.sqrt(np.mean((e - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
     assert errs[1] < errs[0] / 32.0, errs
+
+
+def _quintic_grids(dims, triclinic, overlap=1e30):
+    cell, rect, fpos, ftype, fq, pairs = system(triclinic)
+    ijk = np.array([(i, j, k) for k in range(dims[2]) for j in range(dims[1]) for i in range(dims[0])])
+    shape = (dims[2], dims[1], dims[0], 27)
+    lj = oracle.grid_points_q(cell, rect, dims, 0, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3, ijk, overlap=overlap,
+                              nthreads=8).reshape(shape)
+    q = oracle.grid_points_q(cell, rect, dims, 3, fpos, ftype, fq, pairs, 0, 8.0, 8.0, 0.3, ijk, overlap=overlap,
+                             nthreads=8).reshape(shape)
+    return cell, rect, fpos, ftype, fq, pairs, lj, q
+
+
+@pytest.mark.parametrize("triclinic", [False, True])
+def test_grid_check_quintic_is_exact_at_nodes(triclinic):
+    """The grid test with triquintic grids (system_interpolation_grids.cpp:328-441) at grid nodes: the interpolant
+    reproduces the analytic energy, gradient and Hessian there, so every relative error vanishes; the weights are
+    exp(-beta E) of the independent per-point energy."""
+    dims = (7, 8, 6)
+    cell, rect, fpos, ftype, fq, pairs, lj, q = _quintic_grids(dims, triclinic)
+    nc = np.array(dims) - 1.0
+    frac = np.array([[i, j, k] for i in (1, 3, 5) for j in (2, 4, 6) for k in (1, 2, 4)]) / nc
+    r = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3, lj, frac, 300.0,
+                          coulomb_grid=q, probe_charge=0.8, overlap=1e30)
+    for part in ("lennard_jones", "coulomb"):
+        st = r[part]
+        assert st["energy_relative_error"] < 1e-10
+        assert np.all(st["gradient_relative_error"] < 1e-9) and np.all(st["hessian_relative_error"] < 1e-9), st
+        assert np.allclose(st["hessian_table"], st["hessian_full"], rtol=1e-9, atol=1e-9 * np.abs(st["hessian_full"]).max())
+    ek = np.array([oracle.energy_at_position(cell, rect, 0, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3, f @ cell.T)
+                   for f in frac]) * U["energy_to_kelvin"]
+    w = np.exp(-ek / (U["kB"] * 300.0))
+    assert abs(r["lennard_jones"]["weight"] - w.sum()) <= 1e-12 * w.sum()
+
+
+def test_grid_check_quintic_beats_tricubic_at_random_points():
+    """At random points of the same grid spacing the triquintic energy, gradient and Hessian errors of the grid test
+    are below the tricubic energy and gradient errors (the tricubic scheme reports no Hessian)."""
+    dims = (13, 14, 12)
+    cell, rect, fpos, ftype, fq, pairs, lj, q = _quintic_grids(dims, True, overlap=1e6)
+    lj3 = oracle.grid_build(cell, rect, dims, 0, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3)
+    frac = np.random.default_rng(4).uniform(0, 1, (300, 3))
+    r5 = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3, lj, frac, 300.0)
+    r3 = oracle.grid_check(cell, rect, dims, fpos, ftype, fq, pairs, 2, 8.0, 8.0, 0.3, lj3, frac, 300.0)
+    a, b = r5["lennard_jones"], r3["lennard_jones"]
+    assert a["energy_relative_error"] < b["energy_relative_error"] / 3
+    assert np.all(a["gradient_relative_error"] < b["gradient_relative_error"] / 3)
+    assert np.all(b["hessian_relative_error"] == 0) and np.all(a["hessian_relative_error"] > 0)
+    assert np.all(a["hessian_relative_error"] < 0.5), a["hessian_relative_error"]
