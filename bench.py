@@ -248,11 +248,18 @@ def bench_grids_quintic(ctx, g1, args):
     out["lj_achieved_tflops"] = round(flop_lj / (kms[0] / 1000.0) / 1e12, 3)
     out["lj_frac"] = round(out["lj_achieved_tflops"] / FP64_PEAK_TFLOPS, 4)
     log(f"triquintic grids: ms/step {ms_per_step:.3f} (kernels {kms}); {out['points_per_s']:.4e} grid points/s")
-    # an interpolation spot check of the built LJ grid: Hessians at 1e5 random points (outside the timed region)
-    pts = np.random.default_rng(3).uniform(0, 1, (100000, 3)) @ fw.cell.T
+    # the reference's grid test on the triquintic grids (1e5 points, energies, gradients and Hessians), outside the
+    # timed region
+    pr = spec.index("C_co2")
     t = time.perf_counter()
-    e, _, h = grids[0].interpolate_hessian(pts)
-    out["hessian_points_per_s_host_roundtrip"] = round(len(pts) / (time.perf_counter() - t), 1)
+    c = ra.grid_check(ctx, ff, f, grids[0], pr, grids[2], float(spec.charge[pr]), 300.0, n_points=100000, seed=7)
+    out["grid_check_C_co2"] = dict(
+        lj_energy_relative_error=c["lennard_jones"]["energy_relative_error"],
+        lj_gradient_relative_error=c["lennard_jones"]["gradient_relative_error"].tolist(),
+        lj_hessian_relative_error=c["lennard_jones"]["hessian_relative_error"].tolist(),
+        ewald_energy_relative_error=c["coulomb"]["energy_relative_error"],
+        seconds=round(time.perf_counter() - t, 4))
+    log(f"triquintic grid check: {json.dumps(out['grid_check_C_co2'])}")
     del grids
     return out
 
