@@ -345,21 +345,26 @@ def gcmc_stats(mc, nsys, base, temperature):
                 accepted_insertions=int(acc), insertion_trials=int(tri))
 
 
-def bench_gcmc(ctx, args, framework="itq29", tau=None):
+def bench_gcmc(ctx, args, framework="itq29", tau=None, pressure=1e5):
     """Plain CBMC against the learned first-bead proposal, CO2 at 300 K and 1e5 Pa in charged ITQ-29 (3 x 3 x 3
-    cells) or MFI (2 x 2 x 2 cells).  tau: training temperature factor of the proposal (default args.tau)."""
+    cells), MFI (2 x 2 x 2 cells) or a seeded silicate with two channels (about 5 % of its volume accessible).
+    tau: training temperature factor of the proposal (default args.tau)."""
     import torch
 
     import raspa3_amd as ra
     from raspa3_amd import workloads as w
     from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
 
-    T, P = 300.0, 1e5
+    T, P = 300.0, pressure
     spec = w.forcefield_zeolite()
     tau = args.tau if tau is None else tau
     if framework == "mfi":
         fw, unit = w.framework_mfi(spec, (2, 2, 2))
         nuc, label = 288, "MFI 2x2x2 (2304 sites)"
+    elif framework == "channels":
+        fw = w.framework_channels(spec)
+        unit, nuc = fw.cell, fw.n
+        label = f"a seeded silicate with two 5 A channels (40 x 40 x 30 A, {fw.n} sites)"
     else:
         fw, unit = w.framework_itq29(spec, 3)
         nuc, label = 72, "charged ITQ-29 3x3x3 (1944 sites)"
@@ -403,7 +408,7 @@ def bench_gcmc(ctx, args, framework="itq29", tau=None):
         r.update(seconds=round(dt, 4), accepted_insertions_per_s=r["accepted_insertions"] / dt,
                  insertion_acceptance=r["accepted_insertions"] / max(1, r["insertion_trials"]),
                  kernel_ms_per_cycle=ev0.elapsed_time(ev1) / args.gcmc_production)
-        log(f"gcmc {framework} {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, N = {r['loading']:.3f} +- "
+        log(f"gcmc {framework} {P:g} Pa {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, acc {r['insertion_acceptance']:.4f}, N = {r['loading']:.3f} +- "
             f"{r['loading_err']:.3f}, Qst = {r['qst_kJmol']:.3f} +- {r['qst_err']:.3f} kJ/mol")
         mc.close()
         for s in systems:
@@ -574,6 +579,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--grids-only", action="store_true", help="measure the headline grid build only")
+    ap.add_argument("--gcmc-only", default="", metavar="NAMES",
+                    help="measure the headline grid build and only the GCMC comparisons named (itq29, mfi, channels; "
+                         "name@pressure_pa sets the pressure)")
     ap.add_argument("--quintic-only", action="store_true",
                     help="measure the headline grid build and the triquintic grids of the same workload only")
     ap.add_argument("--widom-insertions", type=int, default=1 << 20)
@@ -618,7 +626,12 @@ def main():
     extra = {"grid_check": g1["grid_check"]}
     if args.quintic_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
-    if not args.grids_only and not args.quintic_only:
+    if args.gcmc_only:
+        for item in args.gcmc_only.split(","):
+            name, _, pa = item.partition("@")
+            extra["gcmc_" + item] = bench_gcmc(ctx, args, name, tau=1.0 if name != "itq29" else None,
+                                               pressure=float(pa) if pa else 1e5)
+    elif not args.grids_only and not args.quintic_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)
