@@ -79,22 +79,21 @@ struct GridArgs
   double* out[MAXG + 1];
 };
 
+// The hardware reciprocal and reciprocal square root are within 5.2e-8 relative (measured over 1e6 arguments in
+// [0.1, 144]); one third-order correction brings them to FP64 rounding: with e the residual, 1/x = y (1 + e + e^2)
+// and 1/sqrt(x) = y (1 + e/2 + 3e^2/8), truncation errors of order e^3 < 1e-21.
 __device__ __forceinline__ double rcp64(double x)
 {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
+  const double y = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, y, 1.0);  // 1 - x y
+  return fma(y, fma(e, e, e), y);
 }
 
 __device__ __forceinline__ double rsq64(double x)
 {
-  double y = __builtin_amdgcn_rsq(x);
-  double h = 0.5 * x;
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+  return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // exp(-y) for y >= 0: Cody-Waite reduction exp(-y) = 2^-n exp(r) with n = rint(y / ln 2) and r = n ln 2 - y formed
