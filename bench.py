@@ -30,6 +30,8 @@ Second half of the metric, in the same JSON line:
   at the simulation temperature; near saturation at 1e5 Pa.
 * gcmc_uneven: batched CBMC over 64 systems of uneven size and loading (ITQ-29 supercells of 1944-7200 sites),
   flexible n-hexane chains and rigid CO2 with the Ewald difference at kmax 8, one launch per cycle.
+* grids_triquintic: the three grids of the headline workload in the triquintic scheme (27 values per point, one
+  launch per grid), per-kernel HIP-event times, FP64 FLOP rates, and the grid test with Hessians.
 """
 import argparse
 import json
