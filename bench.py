@@ -268,6 +268,8 @@ def bench_grids_quintic(ctx, g1, args):
 
 def bench_trial_energies(ctx, g1, args):
     """Widom insertions through the device kernel: trial site energies per second."""
+    import torch
+
     import raspa3_amd as ra
     from raspa3_amd import workloads as w
     from raspa3_amd._lib import RG_GRID_LENNARD_JONES
@@ -285,17 +287,31 @@ def bench_trial_energies(ctx, g1, args):
     comp0 = w.methane_component(spec0)
     p0 = w.mc_params(seed=7)
     n = args.widom_insertions
-    ra.widom(ctx, s0, comp0, p0, 0, 4096)
+    # outputs resident on the device, timed with HIP events on the library's stream; the host-buffer call (results
+    # copied back, as rg_widom returns them) is timed beside it
+    stream = torch.cuda.ExternalStream(ctx.stream, device=torch.device("cuda", 0))
+    outs = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    ptrs = [o.data_ptr() for o in outs]
+    ra.widom_device(ctx, s0, comp0, p0, 0, 4096, *ptrs)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ctx.sync()
+    ev0.record(stream)
+    ra.widom_device(ctx, s0, comp0, p0, 4096, n, *ptrs)
+    ev1.record(stream)
+    ctx.sync()
+    dt = ev0.elapsed_time(ev1) / 1000.0
+    lw = outs[0].cpu().numpy()
     t = time.perf_counter()
-    lw, _, _ = ra.widom(ctx, s0, comp0, p0, 4096, n)
-    dt = time.perf_counter() - t
+    ra.widom(ctx, s0, comp0, p0, 4096, n)
+    dt_host = time.perf_counter() - t
     sites = n * p0["k_first"]
-    out["ch4_config0"] = dict(insertions=n, seconds=round(dt, 4), sites_per_s=sites / dt,
-                              widom_weight=float(np.exp(lw).mean()),
+    out["ch4_config0"] = dict(insertions=n, seconds=round(dt, 5), sites_per_s=sites / dt,
+                              sites_per_s_host_buffers=sites / dt_host, widom_weight=float(np.exp(lw).mean()),
                               roofline=dict(bound="hbm", achieved=sites / dt * GRID_BYTES_PER_SITE / 1e9,
                                             peak=HBM_PEAK_GBS, unit="GB/s",
                                             frac=sites / dt * GRID_BYTES_PER_SITE / 1e9 / HBM_PEAK_GBS))
-    log(f"trial energies CH4 config 0: {sites / dt:.4e} sites/s")
+    log(f"trial energies CH4 config 0: {sites / dt:.4e} sites/s ({sites / dt_host:.4e} with the results copied "
+        "to the host)")
     # configuration 1: rigid CO2 on the three configuration-1 grids, Ewald Fourier difference at kmax 8
     fw1, spec1 = g1["fw"], g1["spec"]
     s1 = ra.System(ctx, g1["ff"], fw1.cell, fw1.rect, framework=g1["f"],
@@ -304,12 +320,15 @@ def bench_trial_energies(ctx, g1, args):
     comp1 = w.co2_component(spec1)
     p1 = w.mc_params(seed=9)
     n1 = max(1024, n // 8)
-    ra.widom(ctx, s1, comp1, p1, 0, 1024)
-    t = time.perf_counter()
-    ra.widom(ctx, s1, comp1, p1, 1024, n1)
-    dt = time.perf_counter() - t
+    ra.widom_device(ctx, s1, comp1, p1, 0, 1024, *ptrs)
+    ctx.sync()
+    ev0.record(stream)
+    ra.widom_device(ctx, s1, comp1, p1, 1024, n1, *ptrs)
+    ev1.record(stream)
+    ctx.sync()
+    dt = ev0.elapsed_time(ev1) / 1000.0
     sites = n1 * (p1["k_first"] + 2 * p1["k_orient"])
-    out["co2_config1"] = dict(insertions=n1, seconds=round(dt, 4), sites_per_s=sites / dt,
+    out["co2_config1"] = dict(insertions=n1, seconds=round(dt, 5), sites_per_s=sites / dt,
                               ewald_deltas_per_s=n1 / dt, n_kvectors=s1.info()["nk"])
     log(f"trial energies CO2 config 1: {sites / dt:.4e} sites/s, {n1 / dt:.4e} Ewald deltas/s")
     out["unit"] = "trial site energies/s"

@@ -313,11 +313,18 @@ typedef struct rg_mc_state_t
 
 typedef struct rg_mc rg_mc;
 
-/* Widom insertions count..count+first-1 into a system (MC_Moves::WidomMove, widom.cpp:32-155): per insertion the
+/* Widom insertions first..first+count-1 into a system (MC_Moves::WidomMove, widom.cpp:32-155): per insertion the
  * log Rosenbluth weight (-inf if every trial overlaps), the external energy of the grown molecule and the Ewald
- * Fourier energy difference.  W = exp(log_weight - beta ewald) / W_IG with W_IG = 1 for rigid bodies. */
+ * Fourier energy difference.  W = exp(log_weight - beta ewald) / W_IG with W_IG = 1 for rigid bodies.  Host
+ * output arrays; returns when they are filled. */
 rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component, const rg_mc_params* params,
                    long long first, int count, double* log_weight, double* energy, double* ewald);
+
+/* Same as rg_widom with device-resident output arrays (count doubles each), for callers that keep the results on
+ * the GPU; returns when the insertions have finished. */
+rg_status rg_widom_device(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,
+                          const rg_mc_params* params, long long first, int count, double* d_log_weight,
+                          double* d_energy, double* d_ewald);
 
 /* GCMC over n_systems independent, initially empty systems sharing one force field; one wavefront per system. */
 rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, int n_components,

@@ -366,6 +366,16 @@ def widom(ctx, system, comp, params, first, count):
     return lw, u, ew
 
 
+def widom_device(ctx, system, comp, params, first, count, log_weight, energy, ewald):
+    """rg_widom_device: the same insertions with device-resident outputs (addresses of count doubles each, e.g.
+    torch tensors' data_ptr()); returns when the kernel has finished."""
+    c = component_desc(comp)
+    p = mc_params(params)
+    L.check(lib().rg_widom_device(ctx.h, system.h, C.byref(c), C.byref(p), int(first), int(count),
+                                  C.c_void_p(int(log_weight)), C.c_void_p(int(energy)), C.c_void_p(int(ewald))),
+            ctx.h, "rg_widom_device")
+
+
 def autotune_summary():
     """Launch configurations chosen by the library's online autotuner, one line per tuned problem."""
     n = lib().rg_autotune_summary(None, 0)

@@ -147,6 +147,8 @@ SIGNATURES.update({
                                 C.POINTER(GridCheck)]),
     "rg_widom": (C.c_int, [vp, vp, C.POINTER(ComponentDesc), C.POINTER(McParams), C.c_longlong, C.c_int, dp, dp,
                            dp]),
+    "rg_widom_device": (C.c_int, [vp, vp, C.POINTER(ComponentDesc), C.POINTER(McParams), C.c_longlong, C.c_int, vp, vp,
+                           vp]),
     "rg_mc_create": (C.c_int, [vp, C.c_int, C.POINTER(vp), C.c_int, C.POINTER(ComponentDesc), C.POINTER(McParams),
                                C.POINTER(vp)]),
     "rg_mc_destroy": (C.c_int, [vp]),

@@ -1048,6 +1048,92 @@ __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, con
   }
 }
 
+// Widom insertions of a single-atom, uncharged component whose pseudo-atom has an interpolation grid: 64 / kFirst
+// insertions per wavefront, one first-bead trial per lane (the general kernel above keeps kFirst lanes of a
+// wavefront busy).  Trial positions come from the same Philox streams as firstBeadStep (system 0, step first + i,
+// trial slot k); the trial energy is the inter-molecular LJ against the system's adsorbates plus the grid value, as
+// in trialSetEnergies; the Rosenbluth weight and the selection follow waveRosenbluth within each group of kFirst
+// lanes, summing in trial order (src/raspakit/cbmc/util.cpp:15-48, multiple_first_bead.cpp:20-76).  The Ewald
+// difference of an uncharged atom is zero.
+__global__ __launch_bounds__(256) void k_widom_single(const McRun* __restrict__ Rp, const SysDev* __restrict__ Sp,
+                                                      const McSys* __restrict__ Yp, int comp, long long first,
+                                                      int count, double* outLogW, double* outU, double* outEwald)
+{
+  const McRun& R = *Rp;
+  const SysDev& S = *Sp;
+  const McSys& Y = *Yp;
+  const Comp& c = R.comp[comp];
+  const int K = R.kFirst;
+  const int G = 64 / K;  // insertions per wavefront
+  const int lane = threadIdx.x & 63;
+  const long long wave = (static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const int j = lane / K, k = lane - j * K;
+  const long long i = wave * G + j;
+  const bool live = j < G && i < count;
+  const int type = c.type[0];
+  double U = 0.0, logBias = 0.0;
+  bool ov = false;
+  unsigned long long step = 0;
+  if (live)
+  {
+    step = static_cast<unsigned long long>(first + i);
+    double p[3];
+    logBias = firstBeadTrial(R, S, Y, 0, step, k, p);
+    for (int a = 0; a < S.nads; ++a)  // the same atom for every lane: uniform loads
+    {
+      const int at = S.at[a];
+      if (at < 0) continue;
+      double dx = S.ax[a] - p[0], dy = S.ay[a] - p[1], dz = S.az[a] - p[2];
+      rgd::pbc(S.box, dx, dy, dz);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < S.rcv2Mol)
+      {
+        const double v = rgd::vdw0(S.pairs[at * S.ntypes + type], 1.0, rr);
+        ov = ov || v > S.overlap;
+        U += v;
+      }
+    }
+    const double g = rgd::interpolate(S.grids[S.gridOfType[type]], p[0], p[1], p[2]);
+    ov = ov || g > S.overlap;
+    U += g;
+  }
+  // Rosenbluth weight and selection of the group, with every lane of the wavefront active (shuffles read only
+  // active lanes): the group's maximum, each trial's weight, then the sum and the cumulative search in trial order
+  const bool valid = live && !ov;
+  const double lbf = valid ? -R.beta * U + logBias : -INFINITY;
+  const unsigned long long vmask = __ballot(valid);
+  const int g0 = min(j, G - 1) * K;  // lanes past the last group follow a real group and discard the result
+  double mx = -INFINITY;
+  for (int t = 0; t < K; ++t) mx = fmax(mx, __shfl(lbf, g0 + t, 64));
+  const double w = valid ? exp(lbf - mx) : 0.0;
+  double uSel1 = 0.0, uSel2 = 0.0;
+  if (live) uni(R, 0, step, SLOT_SELECT, uSel1, uSel2);
+  (void)uSel2;
+  double sum = 0.0;
+  for (int t = 0; t < K; ++t) sum += __shfl(w, g0 + t, 64);
+  const double ws = uSel1 * sum;
+  double pre = 0.0;
+  int sel = -1, lastValid = -1;
+  for (int t = 0; t < K; ++t)
+  {
+    pre += __shfl(w, g0 + t, 64);
+    if ((vmask >> (g0 + t)) & 1ull)
+    {
+      lastValid = t;
+      if (sel < 0 && !(pre < ws)) sel = t;
+    }
+  }
+  if (sel < 0) sel = lastValid;
+  const double Usel = __shfl(U, g0 + max(sel, 0), 64);
+  if (!live || k != 0) return;
+  double logW = -INFINITY;
+  if (lastValid >= 0) logW = mx + log(sum) - log(double(K));
+  const bool ok = lastValid >= 0 && logW >= R.logMinW;
+  outLogW[i] = ok ? logW : -INFINITY;
+  outU[i] = ok ? Usel : 0.0;
+  outEwald[i] = 0.0;
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // GCMC: one wavefront per system, 'cycles' MC cycles per launch.  The wave-uniform state lives in registers (every
 // lane holds the same values) and is written back once by lane 0; the adsorbate arrays, the molecule component

@@ -11,6 +11,8 @@ namespace rg
 {
 template <bool CHAIN>
 __global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample, int nw);
+__global__ void k_widom_single(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first,
+                               int count, double* outLogW, double* outU, double* outEwald);
 template <bool CHAIN>
 __global__ void k_widom(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first, int count,
                         double* outLogW, double* outU, double* outEwald);
@@ -461,6 +463,42 @@ rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out)
   return RG_OK;
 }
 
+// Launches the Widom kernels into device output arrays on the context stream.  'scratch' (runBytes + sysBytes of
+// device memory) receives the run and system descriptors; the caller releases it after the stream has drained.
+static rg_status widomLaunch(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,
+                             const rg_mc_params* params, long long first, int count, double* dLogW, double* dU,
+                             double* dEw, void** scratch)
+{
+  McRun R;
+  rg_status st = buildRun(ctx, sys->ff, 1, component, params, R);
+  if (st != RG_OK) return st;
+  const size_t runBytes = (sizeof(McRun) + 255) / 256 * 256;
+  RG_HIP(ctx, hipMalloc(scratch, runBytes + sizeof(McSys)));
+  McSys Y;
+  std::memset(&Y, 0, sizeof(Y));
+  McRun* dR = static_cast<McRun*>(*scratch);
+  McSys* dY = reinterpret_cast<McSys*>(static_cast<char*>(*scratch) + runBytes);
+  RG_HIP(ctx, hipMemcpyAsync(dR, &R, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(dY, &Y, sizeof(McSys), hipMemcpyHostToDevice, ctx->stream));
+  // a single uncharged atom on a grid: several insertions per wavefront (k_widom_single); otherwise one
+  const int t0 = component->type[0];
+  const bool single = component->n_atoms == 1 && !component->flexible &&
+                      (component->charge[0] == 0.0 || !sys->ff->use_charge) && t0 >= 0 &&
+                      t0 < int(sys->gridOfType.size()) && sys->gridOfType[t0] >= 0 && R.kFirst <= 64;
+  if (single)
+  {
+    const int perWave = 64 / R.kFirst;
+    const long long waves = (static_cast<long long>(count) + perWave - 1) / perWave;
+    hipLaunchKernelGGL(k_widom_single, dim3(static_cast<unsigned>((waves + 3) / 4)), dim3(256), 0, ctx->stream, dR,
+                       sys->d, dY, 0, first, count, dLogW, dU, dEw);
+  }
+  else
+    hipLaunchKernelGGL(component->flexible ? k_widom<true> : k_widom<false>, dim3((count + 3) / 4), dim3(256), 0,
+                       ctx->stream, dR, sys->d, dY, 0, first, count, dLogW, dU, dEw);
+  RG_HIP(ctx, hipGetLastError());
+  return RG_OK;
+}
+
 rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component, const rg_mc_params* params,
                    long long first, int count, double* log_weight, double* energy, double* ewald)
 {
@@ -468,35 +506,45 @@ rg_status rg_widom(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* c
     return fail(ctx, RG_ERR_INVALID, "rg_widom: invalid arguments");
   if (count == 0) return RG_OK;
   RG_ENTER(ctx);
-  McRun R;
-  rg_status st = buildRun(ctx, sys->ff, 1, component, params, R);
-  if (st != RG_OK) return st;
-  void* d = nullptr;
-  const size_t runBytes = (sizeof(McRun) + 255) / 256 * 256, sysBytes = (sizeof(McSys) + 255) / 256 * 256;
-  RG_HIP(ctx, hipMalloc(&d, runBytes + sysBytes + 3 * size_t(count) * sizeof(double)));
-  McSys Y;
-  std::memset(&Y, 0, sizeof(Y));
-  McRun* dR = static_cast<McRun*>(d);
-  McSys* dY = reinterpret_cast<McSys*>(static_cast<char*>(d) + runBytes);
-  double* out = reinterpret_cast<double*>(static_cast<char*>(d) + runBytes + sysBytes);
-  hipError_t e = hipMemcpyAsync(dR, &R, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(dY, &Y, sizeof(McSys), hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-  {
-    hipLaunchKernelGGL(component->flexible ? k_widom<true> : k_widom<false>, dim3((count + 3) / 4), dim3(256), 0,
-                       ctx->stream, dR, sys->d, dY, 0, first, count,
-                       out, out + count, out + 2 * size_t(count));
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(log_weight, out, count * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(energy, out + count, count * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess)
+  double* out = nullptr;
+  void* scratch = nullptr;
+  RG_HIP(ctx, hipMalloc(&out, 3 * size_t(count) * sizeof(double)));
+  rg_status st = widomLaunch(ctx, sys, component, params, first, count, out, out + count, out + 2 * size_t(count),
+                             &scratch);
+  hipError_t e = hipSuccess;
+  if (st == RG_OK) e = hipMemcpyAsync(log_weight, out, count * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == RG_OK && e == hipSuccess)
+    e = hipMemcpyAsync(energy, out + count, count * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == RG_OK && e == hipSuccess)
     e = hipMemcpyAsync(ewald, out + 2 * size_t(count), count * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (st == RG_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(d);
+  (void)hipFree(out);
+  if (scratch) (void)hipFree(scratch);
+  if (st != RG_OK) return st;
   if (e != hipSuccess) return hipfail(ctx, e, "rg_widom");
   return RG_OK;
+}
+
+rg_status rg_widom_device(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,
+                          const rg_mc_params* params, long long first, int count, double* d_log_weight,
+                          double* d_energy, double* d_ewald)
+{
+  if (!ctx || !sys || !component || !params || count < 0 ||
+      (count > 0 && (!d_log_weight || !d_energy || !d_ewald)))
+    return fail(ctx, RG_ERR_INVALID, "rg_widom_device: invalid arguments");
+  if (count == 0) return RG_OK;
+  RG_ENTER(ctx);
+  void* scratch = nullptr;
+  const rg_status st = widomLaunch(ctx, sys, component, params, first, count, d_log_weight, d_energy, d_ewald,
+                                   &scratch);
+  // the descriptors are read by the kernel: released once the stream has passed it
+  if (scratch)
+  {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(scratch);
+  }
+  return st;
 }
 
 rg_status rg_molecule_weights(rg_ctx* ctx, const rg_system* sys, const rg_component_desc* component,

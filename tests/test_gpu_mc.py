@@ -57,6 +57,55 @@ def test_widom_methane_itq29_grid(gpu_ctx):
     assert W > 1.0  # methane adsorbs in the LTA cages
 
 
+def test_widom_methane_into_loaded_system(gpu_ctx):
+    """Widom insertions of CH4 into ITQ-29 holding CH4 molecules from a replayed GCMC run: the single-atom Widom
+    kernel (several insertions per wavefront) with inter-molecular LJ, against the oracle holding the same
+    molecules."""
+    print("case: widom CH4 into a loaded system", flush=True)
+    spec = w.forcefield_zeolite()
+    t = spec.index("CH4")
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, [t])
+    comp = w.methane_component(spec)
+    params = w.mc_params(pressure_pa=1e6, seed=77, max_molecules=256, production_cycles=3)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, kmax=(0, 0, 0), capacity=1024)
+    mc = ra.Mc(gpu_ctx, [s], [comp], params)
+    mc.run(6, sample=False, graph=False)
+    om = oracle.MC(spec, fw, [comp], params, kmax=(0, 0, 0), system=0, grids=(host, None, dims, unit))
+    om.run(6)
+    gp, gt, gc = mc.adsorbates(0)
+    op, ot, oc = om.adsorbates()
+    assert len(gt) == len(ot) > 20 and np.allclose(gp, op, rtol=0, atol=1e-8)
+    loaded = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, kmax=(0, 0, 0), capacity=1024)
+    loaded.set_adsorbates(gp, gt, np.zeros(len(gt)), np.arange(len(gt)))
+    lw, u, ew = ra.widom(gpu_ctx, loaded, comp, params, 1000, 3000)
+    olw, ou, oew = om.widom(0, 1000, 3000)
+    assert close(lw, olw) and close(u, ou) and np.all(ew == 0.0)
+    assert np.isfinite(lw).sum() > 1000
+    mc.close()
+
+
+def test_widom_device_outputs_equal_host_outputs(gpu_ctx):
+    """rg_widom_device writes the same insertions as rg_widom, for the single-atom kernel and the general one.  The
+    device output arrays are the data of a grid object (device memory the library owns and can copy back)."""
+    print("case: widom device outputs", flush=True)
+    spec = w.forcefield_zeolite()
+    types = [spec.index("CH4"), spec.index("C_co2"), spec.index("O_co2")]
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                  capacity=64)
+    params = w.mc_params(seed=3)
+    n = 700
+    buf = ra.Grid(gpu_ctx, unit, True, (7, 7, 7))  # 2744 doubles >= 3 n
+    base = ra.lib().rg_grid_device_data(buf.h)
+    for comp in (w.methane_component(spec), w.co2_reference(spec)):
+        ref = ra.widom(gpu_ctx, s, comp, params, 50, n)
+        buf.upload(np.full(buf.count, 7.0))
+        ra.widom_device(gpu_ctx, s, comp, params, 50, n, base, base + 8 * n, base + 16 * n)
+        got = buf.download(0, 3 * n).reshape(3, n)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+
+
 def test_widom_co2_config1_direct_ewald(gpu_ctx):
     """CO2 Widom in the 4095-site charged triclinic framework: direct framework sums, orientation trials and the
     Ewald Fourier difference at kmax = 8."""
