@@ -183,16 +183,16 @@ def _quintic_case(triclinic):
     return cell, rect, pos, typ, q
 
 
-@pytest.mark.parametrize("triclinic", [False, True])
-def test_triquintic_grids_match_oracle(gpu_ctx, triclinic):
+@pytest.mark.parametrize("triclinic,dims", [(False, (6, 5, 7)), (True, (6, 5, 7)), (True, (5, 2, 7))])
+def test_triquintic_grids_match_oracle(gpu_ctx, triclinic, dims):
     """Triquintic LJ and Ewald real-space grids (27 values per point) at every point of a small framework with a
     supercell, against the oracle's values; per channel the absolute bound also allows 1e-12 of the channel's
     largest magnitude (the moments of the device builder group the same pair terms differently)."""
-    print(f"case: triquintic grids triclinic={triclinic}", flush=True)
+    print(f"case: triquintic grids triclinic={triclinic} dims={dims}", flush=True)
     cell, rect, pos, typ, q = _quintic_case(triclinic)
     fw = w.Framework("small", cell, rect, pos, typ, q)
     spec = w.forcefield_config1()
-    dims = [6, 5, 7]
+    dims = list(dims)
     ff = ra.ForceField(gpu_ctx, spec)
     f = ra.Framework(gpu_ctx, fw)
     gs = [ra.Grid(gpu_ctx, cell, rect, dims, scheme=27) for _ in range(2)]
