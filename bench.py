@@ -592,6 +592,33 @@ def cpu_baseline_gcmc(target_s=10.0):
                        f"give at most C times this")
 
 
+def cpu_baseline_widom(target_s=10.0):
+    """The oracle's Widom insertions of CH4 in configuration 0 (one host thread), the trial-energy workload of the GPU
+    run: grid at 0.2 A, 10 first-bead trials per insertion."""
+    import oracle
+    from raspa3_amd import workloads as w
+
+    fw, spec = w.framework_config0(), w.forcefield_config0()
+    pairs = oracle.pairs_from_spec(spec)
+    t = spec.index("CH4")
+    dims = w.grid_dims(fw.cell, 0.2)[0]
+    g = oracle.grid_build(fw.cell, fw.rect, dims, 0, fw.pos, fw.type, fw.charge, pairs, t, 12.0, 12.0, spec.alpha,
+                          nthreads=host_cores())
+    params = w.mc_params(seed=7)
+    mc = oracle.MC(spec, fw, [w.methane_component(spec)], params, kmax=(0, 0, 0), grids=({t: g}, None, dims))
+    n, dt, first = 0, 0.0, 0
+    while dt < target_s:
+        tt = time.perf_counter()
+        mc.widom(0, first, 20000)
+        dt += time.perf_counter() - tt
+        first += 20000
+        n += 20000
+    sites = n * params["k_first"]
+    return dict(value=sites / dt, unit="trial site energies/s", cores=1, kind="port",
+                sample=f"oracle Widom, CH4 in configuration 0, {n} insertions x {params['k_first']} trials in "
+                       f"{dt:.1f} s; insertions are independent, so C host cores give at most C times this")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -666,6 +693,9 @@ def main():
         if "gcmc" in extra:
             extra["gcmc"]["cpu_baseline"] = cpu_baseline_gcmc()
             log(f"cpu baseline gcmc {extra['gcmc']['cpu_baseline']['value']:.4e} accepted insertions/s")
+        if "trial_energies" in extra:
+            extra["trial_energies"]["cpu_baseline"] = cpu_baseline_widom(5.0)
+            log(f"cpu baseline widom {extra['trial_energies']['cpu_baseline']['value']:.4e} trial site energies/s")
     fw, spec = g1["fw"], g1["spec"]
     out = dict(metric="framework-energy evals/s; GCMC accepted insertions/s at matched loading",
                value=value, unit="grid points/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
