@@ -9,7 +9,8 @@
 // in BF16 for the next product and pre-activations in FP32 for the backward pass.
 //
 // GEMM tile: 64 x 64 outputs per workgroup, four wavefronts of 32 x 32 (2 x 2 MFMA 16 x 16 x 32 blocks), K steps
-// of 32 staged through LDS with the K index contiguous, so every MFMA operand is one 16-byte LDS read.
+// of 32 staged through LDS with the K index contiguous, so every MFMA operand is one 16-byte LDS read; the global
+// loads of the next K step are issued before the products of the current one.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -58,33 +59,58 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int kEnd = min(K, int(blockIdx.z + 1) * Kc);
-  for (int k0 = blockIdx.z * Kc; k0 < kEnd; k0 += 32)
+  // this thread's 8-element slices of the A and B tiles of a K step: read from global memory one step ahead (the
+  // next step's loads are in flight while the current step's products run), then written to LDS
+  auto loadA = [&](int k0) -> bf16x8
   {
     if (!TA)
-    {
-      const int r = t >> 2, c = (t & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&As[r][c]) = *reinterpret_cast<const bf16x8*>(A + size_t(m0 + r) * lda + k0 + c);
-    }
+      return *reinterpret_cast<const bf16x8*>(A + size_t(m0 + (t >> 2)) * lda + k0 + (t & 3) * 8);
+    return *reinterpret_cast<const bf16x8*>(A + size_t(k0 + (t >> 3)) * lda + m0 + (t & 7) * 8);
+  };
+  auto loadB = [&](int k0) -> bf16x8
+  {
+    if (!TB) return *reinterpret_cast<const bf16x8*>(B + size_t(k0 + (t >> 3)) * ldb + n0 + (t & 7) * 8);
+    return *reinterpret_cast<const bf16x8*>(B + size_t(n0 + (t >> 2)) * ldb + k0 + (t & 3) * 8);
+  };
+  auto storeA = [&](const bf16x8& v)
+  {
+    if (!TA)
+      *reinterpret_cast<bf16x8*>(&As[t >> 2][(t & 3) * 8]) = v;
     else
     {
       const int kk = t >> 3, mm = (t & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(A + size_t(k0 + kk) * lda + m0 + mm);
 #pragma unroll
       for (int i = 0; i < 8; ++i) As[mm + i][kk] = v[i];
     }
+  };
+  auto storeB = [&](const bf16x8& v)
+  {
     if (!TB)
     {
       const int kk = t >> 3, nn = (t & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(B + size_t(k0 + kk) * ldb + n0 + nn);
 #pragma unroll
       for (int i = 0; i < 8; ++i) Bs[nn + i][kk] = v[i];
     }
     else
-    {
-      const int r = t >> 2, c = (t & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&Bs[r][c]) = *reinterpret_cast<const bf16x8*>(B + size_t(n0 + r) * ldb + k0 + c);
-    }
+      *reinterpret_cast<bf16x8*>(&Bs[t >> 2][(t & 3) * 8]) = v;
+  };
+  const int kBegin = blockIdx.z * Kc;
+  bf16x8 va, vb;
+  if (kBegin < kEnd)
+  {
+    va = loadA(kBegin);
+    vb = loadB(kBegin);
+  }
+  for (int k0 = kBegin; k0 < kEnd; k0 += 32)
+  {
+    storeA(va);
+    storeB(vb);
     __syncthreads();
+    if (k0 + 32 < kEnd)
+    {
+      va = loadA(k0 + 32);
+      vb = loadB(k0 + 32);
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
     {
