@@ -649,8 +649,9 @@ def main():
                          "byte counters of the traffic figure) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the build path shards over "
-              "z-slabs across GPUs by design, see DESIGN.md, but only the single-GPU case is run)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (a multi-GPU build would shard the "
+              "grids into z-slabs, one per GPU, with no communication; only the single-GPU case is built and run)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
