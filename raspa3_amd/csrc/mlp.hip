@@ -176,10 +176,14 @@ __global__ void k_features(int n, const double* __restrict__ frac, __bf16* __res
   if (i >= n) return;
   for (int q = threadIdx.x; q < NIN; q += blockDim.x)
   {
+    // the phase is reduced to one period in FP64 (m s - floor(m s)), then evaluated in FP32, whose 1e-7 accuracy
+    // is far below the BF16 resolution of the stored feature
     const int a = q / (2 * NFREQ), m = (q % (2 * NFREQ)) / 2 + 1, c = q & 1;
-    const double ph = 2.0 * M_PI * m * frac[3 * i + a];
-    const double v = c ? cos(ph) : sin(ph);
-    X[size_t(i) * NIN + q] = (__bf16)float(v);
+    const double ms = m * frac[3 * i + a];
+    const float ph = float(2.0 * M_PI * (ms - floor(ms)));
+    float sn, cs;
+    sincosf(ph, &sn, &cs);
+    X[size_t(i) * NIN + q] = (__bf16)(c ? cs : sn);
   }
 }
 
