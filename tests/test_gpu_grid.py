@@ -218,6 +218,37 @@ def test_triquintic_grids_match_oracle(gpu_ctx, triclinic, dims):
         np.testing.assert_allclose(e2, e, rtol=1e-13, atol=1e-13 * np.abs(e).max())
 
 
+def test_trial_energies_read_triquintic_grids(gpu_ctx):
+    """Trial-site framework energies (rg_trial_energies) from triquintic LJ and Coulomb grids equal the oracle's
+    triquintic interpolation of the same grid data (Coulomb scaled by the site charge)."""
+    print("case: trial energies on triquintic grids", flush=True)
+    cell, rect, pos, typ, q = _quintic_case(True)
+    fw = w.Framework("small", cell, rect, pos, typ, q)
+    spec = w.forcefield_config1()
+    dims = [9, 10, 11]
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    probe = spec.index("C_co2")
+    lj, coul = ra.Grid(gpu_ctx, cell, rect, dims, 27), ra.Grid(gpu_ctx, cell, rect, dims, 27)
+    ra.build_grids(gpu_ctx, ff, f, [(lj, RG_GRID_LENNARD_JONES, probe, 12.0), (coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    s = ra.System(gpu_ctx, ff, cell, rect, framework=f, grids={probe: lj}, coulomb_grid=coul, kmax=(0, 0, 0),
+                  capacity=16)
+    rng = np.random.default_rng(12)
+    n = 500
+    sp = rng.uniform(0, 1, (n, 3)) @ cell.T
+    st = np.full(n, probe, np.int32)
+    sq = np.full(n, 0.7)
+    e, ov = s.trial_energies(sp, st, sq, np.full(n, 1000, np.int32))
+    shape = (dims[2], dims[1], dims[0], 27)
+    elj = oracle.interpolate_q(cell, rect, dims, lj.download().reshape(shape), sp)[0]
+    eq = 0.7 * oracle.interpolate_q(cell, rect, dims, coul.download().reshape(shape), sp)[0]
+    ok = ~np.asarray(ov, bool)
+    assert ok.sum() > n // 2
+    assert np.allclose(e[ok, 0], elj[ok], rtol=1e-9, atol=1e-9)
+    assert np.allclose(e[ok, 1], eq[ok], rtol=1e-9, atol=1e-9)
+    assert np.all(elj[~ok] > spec.overlap)
+
+
 def test_mixed_schemes_are_rejected(gpu_ctx):
     cell, rect, pos, typ, q = _quintic_case(True)
     fw = w.Framework("small", cell, rect, pos, typ, q)
