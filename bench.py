@@ -30,6 +30,10 @@ Second half of the metric, in the same JSON line:
   at the simulation temperature; near saturation at 1e5 Pa.
 * gcmc_uneven: batched CBMC over 64 systems of uneven size and loading (ITQ-29 supercells of 1944-7200 sites),
   flexible n-hexane chains and rigid CO2 with the Ewald difference at kmax 8, one launch per cycle.
+* gcmc_mixture: CO2/N2 80:20 GCMC at 300 K and 1e5 Pa in 256 systems of uneven size (charged ITQ-29 supercells of
+  2592-15552 sites, Ewald kmax 8), one launch per cycle, plain CBMC against the learned first-bead proposal (trained
+  on the unit cell, tiled over each supercell): accepted insertions/s, loading per cell of both components and the
+  isosteric heat, and whether they agree within 2 sigma.
 * grids_triquintic: the three grids of the headline workload in the triquintic scheme (27 values per point, one
   launch per grid), per-kernel HIP-event times, FP64 FLOP rates, and the grid test with Hessians.
 """
@@ -531,6 +535,142 @@ def bench_gcmc_uneven(ctx, args):
                 max_energy_drift=float(drift), unit="accepted insertions/s")
 
 
+def bench_gcmc_mixture(ctx, args):
+    """Combined workload: CO2/N2 80:20 GCMC in many systems of uneven size (charged ITQ-29 supercells of 2592-15552
+    sites), one launch per cycle, plain CBMC against the learned first-bead proposal.  The proposal network is
+    trained on CO2 in the 11.87 A unit cell and its table is tiled over each supercell; it proposes the first bead of
+    both components (any positive density keeps the acceptance rule exact)."""
+    import raspa3_amd as ra
+    from raspa3_amd import workloads as w
+    from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
+
+    T, P, v = 300.0, args.mixture_pressure, args.mixture_voxels
+    spec = w.forcefield_zeolite([("N_n2", 36.0, 3.31, -0.482), ("N_com", 0.0, 1.0, 0.964)])
+    ff = ra.ForceField(ctx, spec)
+    fw3, unit = w.framework_itq29(spec, 3)
+    fu = ra.Framework(ctx, w.Framework("itq29", unit, True, fw3.pos[:72], fw3.type[:72], fw3.charge[:72]))
+    dims = ra.grid_dims(unit, 0.2)
+    grids = {spec.index(n): ra.Grid(ctx, unit, True, dims) for n in ("C_co2", "O_co2", "N_n2", "N_com")}
+    coul = ra.Grid(ctx, unit, True, dims)
+    ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
+                   [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
+    comps = [w.co2_reference(spec, 0.8), w.n2_component(spec, 0.2)]
+    shapes = [(3, 3, 4), (3, 4, 4), (4, 4, 4), (4, 4, 5), (4, 5, 5), (5, 5, 5), (5, 5, 6), (5, 6, 6), (6, 6, 6)]
+    frameworks = {}
+    for sh in shapes:
+        fwk, _ = w.framework_itq29(spec, sh)
+        frameworks[sh] = (fwk, ra.Framework(ctx, fwk))
+    nsys, mmax = args.mixture_systems, args.mixture_max_molecules
+    shape_of = [shapes[k % len(shapes)] for k in range(nsys)]
+    cells = np.array([np.prod(sh) for sh in shape_of], float)
+
+    # proposal: orientation-averaged CO2 Boltzmann factors at 2^20 points of the unit cell, tempered, tabulated on
+    # v^3 voxels of the unit cell (x fastest) and repeated over each supercell
+    s0 = ra.System(ctx, ff, unit, True, framework=fu, grids=grids, coulomb_grid=coul, kmax=(0, 0, 0), capacity=64)
+    frac = np.random.default_rng(6).random((1 << 20, 3))
+    lw = ra.molecule_weights(ctx, s0, comps[0], T, frac, 16, 13)
+    s0.close()
+    beta = ra.beta(T)
+    net = ra.Mlp(ctx, seed=4)
+    t = time.perf_counter()
+    losses = net.train(frac, np.where(np.isfinite(lw), -lw / beta, 1e7), args.tau * beta, batch=4096,
+                       epochs=args.mlp_epochs, lr=1e-3, seed=10)
+    t_train = time.perf_counter() - t
+    cell_table = net.proposal([v, v, v], args.floor).reshape(v, v, v)
+    net.close()
+    tables = {sh: ([v * sh[0], v * sh[1], v * sh[2]], np.tile(cell_table, (sh[2], sh[1], sh[0])).reshape(-1))
+              for sh in shapes}
+
+    def run(label, seed, learned):
+        systems = [ra.System(ctx, ff, frameworks[sh][0].cell, True, framework=frameworks[sh][1], grids=grids,
+                             coulomb_grid=coul, kmax=(8, 8, 8), capacity=3 * mmax) for sh in shape_of]
+        params = w.mc_params(temperature=T, pressure_pa=P, seed=seed, max_molecules=mmax,
+                             production_cycles=args.mixture_production)
+        mc = ra.Mc(ctx, systems, comps, params)
+        if learned:
+            for k, sh in enumerate(shape_of):
+                mc.set_proposal(k, *tables[sh])
+        mc.run(args.mixture_equilibration, sample=False, graph=True)
+        log(f"gcmc mixture {label}: {args.mixture_equilibration} equilibration cycles done")
+        st0 = [mc.state(k) for k in range(nsys)]
+        ctx.sync()
+        t = time.perf_counter()
+        mc.run(args.mixture_production, sample=True, graph=True)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        st1 = [mc.state(k) for k in range(nsys)]
+        acc = sum(st1[k]["accepted"][0] - st0[k]["accepted"][0] for k in range(nsys))
+        tri = sum(st1[k]["trials"][0] - st0[k]["trials"][0] for k in range(nsys))
+        per_cell, rows = [], []
+        for k, st in enumerate(st1):
+            c = st["block_count"].sum()
+            per_cell.append(st["block_n"].sum(0) / c / cells[k])
+            rows.append([c, st["block_energy"].sum(), st["block_n"].sum(), st["block_energy_n"].sum(),
+                         st["block_nn"].sum()])
+        rows = np.array(rows)
+
+        def q_of(r):
+            cnt, U, N, UN, NN = r
+            return (-(UN / cnt - U / cnt * N / cnt) / (NN / cnt - (N / cnt) ** 2) * ra.ENERGY_TO_KELVIN
+                    + T) * 8.314462618e-3
+
+        # isosteric heat of the total loading: fluctuation formula on the moments pooled over the systems of one
+        # supercell shape (a per-system ratio is biased at a few effectively independent samples), jackknife over
+        # those systems, then the mean over the shapes
+        q_shape, e2_shape = [], []
+        for sh in shapes:
+            g = rows[[k for k in range(nsys) if shape_of[k] == sh]]
+            if len(g) < 2:
+                continue
+            tot = g.sum(0)
+            jk = np.array([q_of(tot - r) for r in g])
+            q_shape.append(q_of(tot))
+            e2_shape.append((len(g) - 1) / len(g) * ((jk - jk.mean()) ** 2).sum())
+        # CO2 per cell in each block of the production cycles (mean over systems): flat once equilibrated
+        by_block = np.mean([st["block_n"][:, 0] / np.maximum(st["block_count"], 1) / cells[k]
+                            for k, st in enumerate(st1)], axis=0)
+        per_cell = np.array(per_cell)
+        tot = mc.total_energy()
+        drift = max(abs(tot[k] - st1[k]["energy"]) for k in range(nsys))
+        n_final = np.array([st["n_per_component"] for st in st1])
+        mc.close()
+        for s in systems:
+            s.close()
+        r = dict(loading_per_cell=per_cell.mean(0).tolist(),
+                 loading_per_cell_err=(per_cell.std(0, ddof=1) / np.sqrt(nsys)).tolist(),
+                 qst_kJmol=float(np.mean(q_shape)), qst_err=float(np.sqrt(np.sum(e2_shape)) / len(q_shape)),
+                 qst_by_shape=[round(float(x), 3) for x in q_shape],
+                 accepted_insertions=int(acc), insertion_trials=int(tri), seconds=round(dt, 4),
+                 accepted_insertions_per_s=acc / dt, insertion_acceptance=acc / max(1, tri),
+                 co2_per_cell_by_block=[round(float(x), 4) for x in by_block],
+                 molecules_final=[int(n_final.sum(1).min()), int(n_final.sum(1).max())],
+                 max_energy_drift=float(drift))
+        log(f"gcmc mixture {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, acc "
+            f"{r['insertion_acceptance']:.4f}, CO2/N2 per cell {r['loading_per_cell'][0]:.4f}/"
+            f"{r['loading_per_cell'][1]:.4f}, Qst {r['qst_kJmol']:.3f} +- {r['qst_err']:.3f} kJ/mol")
+        return r
+
+    plain = run("plain", 2001, False)
+    learned = run("learned", 2002, True)
+
+    def within(a, ea, b, eb):
+        return bool(abs(a - b) <= 2.0 * np.hypot(ea, eb))
+
+    return dict(workload=f"CO2/N2 80:20 at {T:g} K and {P:g} Pa, {nsys} systems in charged ITQ-29 supercells "
+                         f"{shapes} (2592-15552 sites), Ewald kmax 8, {args.mixture_equilibration} + "
+                         f"{args.mixture_production} cycles, grids at 0.2 A on the unit cell",
+                plain=plain, learned=learned,
+                learned_over_plain=learned["accepted_insertions_per_s"] / plain["accepted_insertions_per_s"],
+                loading_within_2sigma=[within(learned["loading_per_cell"][i], learned["loading_per_cell_err"][i],
+                                              plain["loading_per_cell"][i], plain["loading_per_cell_err"][i])
+                                       for i in range(2)],
+                qst_within_2sigma=within(learned["qst_kJmol"], learned["qst_err"], plain["qst_kJmol"],
+                                         plain["qst_err"]),
+                proposal=dict(training_points=len(frac), voxels_per_unit_cell=v, tau=args.tau, floor=args.floor,
+                              epochs=args.mlp_epochs, final_loss=float(losses[-1]), train_seconds=round(t_train, 4)),
+                unit="accepted insertions/s")
+
+
 def bench_mlp(ctx, g1, args):
     """Proposal-network training throughput on configuration-1 CO2 grid data (2^21 points, 300 K)."""
     import raspa3_amd as ra
@@ -644,6 +784,14 @@ def main():
     ap.add_argument("--uneven-systems", type=int, default=64)
     ap.add_argument("--uneven-equilibration", type=int, default=300)
     ap.add_argument("--uneven-production", type=int, default=100)
+    ap.add_argument("--mixture-only", action="store_true",
+                    help="measure the headline grid build and the CO2/N2 mixture GCMC over uneven systems only")
+    ap.add_argument("--mixture-systems", type=int, default=256)
+    ap.add_argument("--mixture-equilibration", type=int, default=2000)
+    ap.add_argument("--mixture-production", type=int, default=800)
+    ap.add_argument("--mixture-pressure", type=float, default=1e5)
+    ap.add_argument("--mixture-voxels", type=int, default=16, help="proposal voxels per unit-cell axis")
+    ap.add_argument("--mixture-max-molecules", type=int, default=512)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 3 times (the reference kernel that scales the HBM "
                          "byte counters of the traffic figure) and exit")
@@ -680,6 +828,8 @@ def main():
             name, _, pa = item.partition("@")
             extra["gcmc_" + item] = bench_gcmc(ctx, args, name, tau=1.0 if name != "itq29" else None,
                                                pressure=float(pa) if pa else 1e5)
+    elif args.mixture_only:
+        extra["gcmc_mixture"] = bench_gcmc_mixture(ctx, args)
     elif not args.grids_only and not args.quintic_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
@@ -687,6 +837,7 @@ def main():
         extra["gcmc"] = bench_gcmc(ctx, args)
         extra["gcmc_mfi"] = bench_gcmc(ctx, args, "mfi", tau=1.0)
         extra["gcmc_uneven"] = bench_gcmc_uneven(ctx, args)
+        extra["gcmc_mixture"] = bench_gcmc_mixture(ctx, args)
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(g1["fw"], g1["spec"], g1["dims"], args.cpu_seconds)

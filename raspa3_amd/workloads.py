@@ -207,9 +207,13 @@ def co2_component(spec, mol_fraction=1.0):
 
 
 def n2_component(spec, mol_fraction=1.0):
-    """N2 of configuration 4: two LJ sites (36 K, 3.31 A, -0.482 e) 0.55 A either side of a +0.964 e centre;
-    first bead = the centre."""
-    return dict(n_atoms=3, first_bead=1, position=[[0, 0, 0.55], [0, 0, 0], [0, 0, -0.55]],
+    """N2 of configuration 4: two LJ sites (36 K, 3.31 A, -0.482 e) 0.55 A either side of a +0.964 e centre
+    (examples/basic/2_mc_co2_n2_in_two_independent_boxes/N2.json).  First bead = atom 0, an N site: the reference's
+    default CBMC starting bead (src/raspakit/component.ixx:195).  Growing from the charge-only centre instead is
+    exact too but samples badly: a centre trial beside a framework O scores about -87,000 K on its own (the N sites
+    that keep the whole molecule away are placed only in the orientation step), so a few first-bead weights of
+    order e^290 at 300 K dominate the Rosenbluth factors."""
+    return dict(n_atoms=3, first_bead=0, position=[[0, 0, 0.55], [0, 0, 0], [0, 0, -0.55]],
                 type=[spec.index("N_n2"), spec.index("N_com"), spec.index("N_n2")], charge=[-0.482, 0.964, -0.482],
                 mol_fraction=mol_fraction)
 
