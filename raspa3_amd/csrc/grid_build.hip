@@ -1340,10 +1340,10 @@ rg_status rg_grid_create_scheme(rg_ctx* ctx, const double* cell9, int is_rectang
 rg_status rg_grid_destroy(rg_grid* g)
 {
   if (!g) return RG_ERR_INVALID;
-  hipSetDevice(g->device);
-  if (g->d) hipFree(g->d);
+  rg_status st = rgReleaseStatus(hipSetDevice(g->device), RG_OK);
+  if (g->d) st = rgReleaseStatus(hipFree(g->d), st);
   delete g;
-  return RG_OK;
+  return st;
 }
 
 void* rg_grid_device_data(rg_grid* g) { return g ? g->d : nullptr; }

@@ -267,12 +267,12 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
 rg_status rg_system_destroy(rg_system* s)
 {
   if (!s) return RG_ERR_INVALID;
-  (void)hipSetDevice(s->device);
+  rg_status st = rgReleaseStatus(hipSetDevice(s->device), RG_OK);
   // the context (and its stream) may already be gone: wait for the whole device instead
-  (void)hipDeviceSynchronize();
-  for (void* p : s->owned) (void)hipFree(p);
+  st = rgReleaseStatus(hipDeviceSynchronize(), st);
+  for (void* p : s->owned) st = rgReleaseStatus(hipFree(p), st);
   delete s;
-  return RG_OK;
+  return st;
 }
 
 rg_status rg_system_info(const rg_system* s, int* nk, double* single_ion, double* net_charge, int* n_adsorbate_atoms)

@@ -1,5 +1,6 @@
 // Host side of the Monte Carlo engine: multi-system GCMC state, proposal tables, hipGraph-replayed cycles, Widom.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -34,6 +35,7 @@ struct rg_mc
   McSys* dY = nullptr;
   std::vector<McSys> hY;  // host mirror of the static parts (pointers)
   std::vector<void*> owned;
+  std::vector<std::array<void*, 3>> proposalTables;  // per system: alias probabilities, aliases, log(nvox p)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
   int graphSample = -1;
@@ -208,23 +210,30 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
 rg_status rg_mc_destroy(rg_mc* mc)
 {
   if (!mc) return RG_ERR_INVALID;
-  (void)hipSetDevice(mc->device);
+  rg_status st = rgReleaseStatus(hipSetDevice(mc->device), RG_OK);
   // the context (and its stream) may already be gone: wait for the whole device instead
-  (void)hipDeviceSynchronize();
+  st = rgReleaseStatus(hipDeviceSynchronize(), st);
   // the systems already hold the final adsorbate count and net charge (handed back after every run), so they may
   // have been released before the engine
-  if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
-  if (mc->graph) (void)hipGraphDestroy(mc->graph);
-  for (void* p : mc->owned) (void)hipFree(p);
+  if (mc->graphExec) st = rgReleaseStatus(hipGraphExecDestroy(mc->graphExec), st);
+  if (mc->graph) st = rgReleaseStatus(hipGraphDestroy(mc->graph), st);
+  for (void* p : mc->owned) st = rgReleaseStatus(hipFree(p), st);
+  for (const auto& t : mc->proposalTables)
+    for (void* p : t)
+      if (p) st = rgReleaseStatus(hipFree(p), st);
   delete mc;
-  return RG_OK;
+  return st;
 }
 
 rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability)
 {
-  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || !n3)
+  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || (probability && !n3))
     return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: invalid arguments");
+  RG_ENTER(ctx);
   McSys& Y = mc->hY[system];
+  if (mc->proposalTables.size() < mc->systems.size()) mc->proposalTables.resize(mc->systems.size(), {{nullptr, nullptr, nullptr}});
+  const std::array<void*, 3> replaced = mc->proposalTables[system];
+  std::array<void*, 3> fresh = {{nullptr, nullptr, nullptr}};
   if (!probability)
   {
     Y.prop = Proposal{};
@@ -266,13 +275,23 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
     }
     for (int v : large) prob[v] = 1.0, alias[v] = v;
     for (int v : small) prob[v] = 1.0, alias[v] = v;
-    void *dp, *da, *dl;
-    RG_HIP(ctx, hipMalloc(&dp, nvox * sizeof(double)));
-    mc->owned.push_back(dp);
-    RG_HIP(ctx, hipMalloc(&da, nvox * sizeof(int)));
-    mc->owned.push_back(da);
-    RG_HIP(ctx, hipMalloc(&dl, nvox * sizeof(double)));
-    mc->owned.push_back(dl);
+    auto release = [&fresh]() {
+      for (void* p : fresh)
+        if (p) (void)hipFree(p);
+    };
+    const size_t bytes[3] = {nvox * sizeof(double), nvox * sizeof(int), nvox * sizeof(double)};
+    for (int t = 0; t < 3; ++t)
+    {
+      const hipError_t e = hipMalloc(&fresh[t], bytes[t]);
+      if (e != hipSuccess)
+      {
+        fresh[t] = nullptr;
+        release();
+        return hipfail(ctx, e, "rg_mc_set_proposal: hipMalloc");
+      }
+    }
+    void *dp = fresh[0], *da = fresh[1], *dl = fresh[2];
+    mc->proposalTables[system] = fresh;
     RG_HIP(ctx, hipMemcpyAsync(dp, prob.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     RG_HIP(ctx, hipMemcpyAsync(da, alias.data(), nvox * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     RG_HIP(ctx, hipMemcpyAsync(dl, logp.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -283,9 +302,13 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
     Y.prop.logNvoxP = static_cast<const double*>(dl);
   }
   // only the proposal descriptor changes: copy it into the device state
+  if (!probability) mc->proposalTables[system] = fresh;
   RG_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char*>(mc->dY + system) + offsetof(McSys, prop), &Y.prop,
                              sizeof(Proposal), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // no kernel reads the replaced tables any more: the device descriptor points at the new ones
+  for (void* p : replaced)
+    if (p) RG_HIP(ctx, hipFree(p));
   return RG_OK;
 }
 

@@ -535,15 +535,15 @@ rg_status rg_mlp_create(rg_ctx* ctx, unsigned long long seed, rg_mlp** out)
 rg_status rg_mlp_destroy(rg_mlp* p)
 {
   if (!p) return RG_ERR_INVALID;
-  (void)hipSetDevice(p->device);
+  rg_status st = rgReleaseStatus(hipSetDevice(p->device), RG_OK);
   // the context (and its stream) may already be gone: wait for the whole device instead
-  (void)hipDeviceSynchronize();
+  st = rgReleaseStatus(hipDeviceSynchronize(), st);
   freeBatch(p);
   void* bufs[] = {p->w, p->wb, p->g, p->m, p->v};
   for (void* b : bufs)
-    if (b) (void)hipFree(b);
+    if (b) st = rgReleaseStatus(hipFree(b), st);
   delete p;
-  return RG_OK;
+  return st;
 }
 
 rg_status rg_mlp_get_params(rg_ctx* ctx, const rg_mlp* p, float* out)

@@ -41,6 +41,8 @@ rg_status fail(rg_ctx* ctx, rg_status s, const char* fmt, ...)
 
 rg_status hipfail(rg_ctx* ctx, hipError_t e, const char* what)
 {
+  // the error is reported here: clear HIP's last-error state so that the next entry point does not report it again
+  (void)hipGetLastError();
   return fail(ctx, e == hipErrorOutOfMemory ? RG_ERR_NOMEM : RG_ERR_DEVICE, "%s failed: %s", what,
               hipGetErrorString(e));
 }
@@ -119,15 +121,15 @@ rg_status rg_ctx_create(int device, rg_ctx** out)
 rg_status rg_ctx_destroy(rg_ctx* ctx)
 {
   if (!ctx) return RG_ERR_INVALID;
-  hipSetDevice(ctx->device);
+  rg_status st = rgReleaseStatus(hipSetDevice(ctx->device), RG_OK);
   if (ctx->stream)
   {
-    hipStreamSynchronize(ctx->stream);
-    hipStreamDestroy(ctx->stream);
+    st = rgReleaseStatus(hipStreamSynchronize(ctx->stream), st);
+    st = rgReleaseStatus(hipStreamDestroy(ctx->stream), st);
   }
-  if (ctx->d_table) hipFree(ctx->d_table);
+  if (ctx->d_table) st = rgReleaseStatus(hipFree(ctx->d_table), st);
   delete ctx;
-  return RG_OK;
+  return st;
 }
 
 const char* rg_last_error(const rg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -297,11 +299,11 @@ rg_status rg_framework_create(rg_ctx* ctx, const double* cell9, int is_rectangul
 rg_status rg_framework_destroy(rg_framework* fw)
 {
   if (!fw) return RG_ERR_INVALID;
-  hipSetDevice(fw->device);
-  if (fw->images.d_img) hipFree(fw->images.d_img);
-  if (fw->images.d_bin_start) hipFree(fw->images.d_bin_start);
+  rg_status st = rgReleaseStatus(hipSetDevice(fw->device), RG_OK);
+  if (fw->images.d_img) st = rgReleaseStatus(hipFree(fw->images.d_img), st);
+  if (fw->images.d_bin_start) st = rgReleaseStatus(hipFree(fw->images.d_bin_start), st);
   delete fw;
-  return RG_OK;
+  return st;
 }
 
 }  // extern "C"

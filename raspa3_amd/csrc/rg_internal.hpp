@@ -96,12 +96,18 @@ void widths3(const double* m, double* w);
     if (rg_e_ != hipSuccess) return rg::hipfail(ctx, rg_e_, #call); \
   } while (0)
 
-// Entry of an API call: makes the context's device current and clears the error state that an earlier unchecked
-// HIP call (a release path, another library) may have left, so that the hipGetLastError after this call's own
-// launches reports this call's errors only.
-#define RG_ENTER(ctx)                           \
-  do                                            \
-  {                                             \
-    RG_HIP(ctx, hipSetDevice((ctx)->device));   \
-    (void)hipGetLastError();                    \
+// Entry of an API call: makes the context's device current and reports an error that an earlier HIP call (another
+// library, or an unchecked release path) left pending, instead of letting the hipGetLastError after this call's own
+// launches attribute it to this call.  Reading the error clears it, so the caller may retry.
+#define RG_ENTER(ctx)                                                                                  \
+  do                                                                                                   \
+  {                                                                                                    \
+    RG_HIP(ctx, hipSetDevice((ctx)->device));                                                          \
+    const hipError_t rg_pending_ = hipGetLastError();                                                  \
+    if (rg_pending_ != hipSuccess)                                                                     \
+      return rg::fail(ctx, RG_ERR_DEVICE, "%s: a HIP error was pending before this call: %s", __func__, \
+                      hipGetErrorString(rg_pending_));                                                 \
   } while (0)
+
+// Status of a release path: the first failing HIP call of a destroy function (the object is released anyway).
+inline rg_status rgReleaseStatus(hipError_t e, rg_status st) { return (st == RG_OK && e != hipSuccess) ? RG_ERR_DEVICE : st; }
