@@ -237,6 +237,17 @@ rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* sys, int n, const doub
                             const double* charge, const int* molecule, int skip_molecule, double* energies,
                             int* overlap);
 
+/* No reference counterpart: rg_trial_energies for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight Monte Carlo or serving loops, harnesses that forbid hipMalloc).  Same arguments, results and errors,
+ * but the device staging of the inputs and results lives in a static array of the library (one per device, held until
+ * the process ends) instead of a per-call allocation, so n may not exceed RG_TRIAL_NOALLOC_MAX_SITES (RG_ERR_INVALID
+ * beyond it).  The array is shared by every call on a device: calls that overlap on one device (from several host
+ * threads or contexts) race and must be serialised by the caller. */
+#define RG_TRIAL_NOALLOC_MAX_SITES 16384
+rg_status rg_trial_energies_noalloc(rg_ctx* ctx, const rg_system* sys, int n, const double* pos, const int* type,
+                                    const double* charge, const int* molecule, int skip_molecule, double* energies,
+                                    int* overlap);
+
 /* Rosenbluth weight and selection for n_sets sets of k trials (src/raspakit/cbmc/multiple_first_bead.cpp:20-49,
  * src/raspakit/cbmc/util.cpp:15-48): log_weight[s] = log(sum_i exp(-beta U_i) / k) over non-overlapping trials,
  * selected[s] = the trial the cumulative search picks for uniforms[s] (-1 if all overlap). */

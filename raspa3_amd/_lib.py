@@ -57,6 +57,7 @@ SIGNATURES = {
     "rg_system_info": (C.c_int, [vp, ip, dp, dp, ip]),
     "rg_system_set_adsorbates": (C.c_int, [vp, vp, C.c_int, dp, ip, dp, ip]),
     "rg_trial_energies": (C.c_int, [vp, vp, C.c_int, dp, ip, dp, ip, C.c_int, dp, ip]),
+    "rg_trial_energies_noalloc": (C.c_int, [vp, vp, C.c_int, dp, ip, dp, ip, C.c_int, dp, ip]),
     "rg_rosenbluth_select": (C.c_int, [vp, C.c_int, C.c_int, dp, ip, C.c_double, dp, ip, dp]),
     "rg_ewald_delta": (C.c_int, [vp, vp, C.c_int, dp, dp, ip, C.c_int, dp, dp, ip, dp]),
     "rg_ewald_accept": (C.c_int, [vp, vp]),

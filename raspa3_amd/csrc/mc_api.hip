@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mc_internal.hpp"
@@ -327,19 +328,21 @@ rg_status rg_system_set_adsorbates(rg_ctx* ctx, rg_system* s, int n, const doubl
   return RG_OK;
 }
 
-rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* s, int n, const double* pos, const int* type,
-                            const double* q, const int* molecule, int skip_molecule, double* energies, int* overlap)
+}  // extern "C"
+
+namespace
 {
-  if (!ctx || !s || n < 0 || (n > 0 && (!pos || !type || !q || !molecule || !energies || !overlap)))
-    return fail(ctx, RG_ERR_INVALID, "rg_trial_energies: invalid arguments");
-  if (n == 0) return RG_OK;
-  for (int i = 0; i < n; ++i)
-    if (type[i] < 0 || type[i] >= s->h.ntypes) return fail(ctx, RG_ERR_INVALID, "rg_trial_energies: type out of range");
-  RG_ENTER(ctx);
-  Scratch sc{ctx};
-  const size_t bytes = size_t(n) * (3 * 8 + 4 + 8 + 4 + 4 * 8 + 4);
-  RG_HIP(ctx, hipMalloc(&sc.p, bytes));
-  char* p = static_cast<char*>(sc.p);
+
+// Bytes of device staging per trial site: position (3 doubles), charge, the four energies, type, molecule, overlap.
+constexpr size_t kTrialStageBytes = 3 * 8 + 8 + 4 * 8 + 4 + 4 + 4;
+
+// Stages the host inputs of n trial sites into 'stage' (n * kTrialStageBytes bytes of device memory), evaluates their
+// energies and copies the results back; returns when they are on the host.
+rg_status trialEnergiesStaged(rg_ctx* ctx, const rg_system* s, int n, const double* pos, const int* type,
+                              const double* q, const int* molecule, int skip_molecule, double* energies, int* overlap,
+                              void* stage)
+{
+  char* p = static_cast<char*>(stage);
   double* dpos = reinterpret_cast<double*>(p);
   double* dq = dpos + 3 * size_t(n);
   double* dout = dq + n;
@@ -357,6 +360,72 @@ rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* s, int n, const double
   RG_HIP(ctx, hipMemcpyAsync(overlap, dov, size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RG_OK;
+}
+
+rg_status trialArgs(rg_ctx* ctx, const rg_system* s, int n, const double* pos, const int* type, const double* q,
+                    const int* molecule, double* energies, int* overlap, const char* who)
+{
+  if (!ctx || !s || n < 0 || (n > 0 && (!pos || !type || !q || !molecule || !energies || !overlap)))
+    return fail(ctx, RG_ERR_INVALID, "%s: invalid arguments", who);
+  for (int i = 0; i < n; ++i)
+    if (type[i] < 0 || type[i] >= s->h.ntypes) return fail(ctx, RG_ERR_INVALID, "%s: type out of range", who);
+  return RG_OK;
+}
+
+// Staging of rg_trial_energies_noalloc: one array per device, held until the process ends.  Host code never uses the
+// name 'scratch' as an address (on the host it is only a handle for the symbol lookup); it uses the device address
+// that hipGetSymbolAddress returns for the current device, looked up once per device.
+static __device__ __align__(256) char scratch[RG_TRIAL_NOALLOC_MAX_SITES * kTrialStageBytes];
+
+std::mutex gScratchMutex;
+std::vector<void*> gScratchPtr;  // per device: device address of 'scratch', or null before the first lookup
+
+rg_status scratchAddress(rg_ctx* ctx, void** out)
+{
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (ctx->device >= int(gScratchPtr.size())) gScratchPtr.resize(ctx->device + 1, nullptr);
+  if (!gScratchPtr[ctx->device])
+  {
+    void* ptr = nullptr;
+    RG_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(ctx, RG_ERR_DEVICE, "rg_trial_energies_noalloc: no device address for the staging array");
+    gScratchPtr[ctx->device] = ptr;
+  }
+  *out = gScratchPtr[ctx->device];
+  return RG_OK;
+}
+
+}  // namespace
+
+extern "C"
+{
+
+rg_status rg_trial_energies(rg_ctx* ctx, const rg_system* s, int n, const double* pos, const int* type,
+                            const double* q, const int* molecule, int skip_molecule, double* energies, int* overlap)
+{
+  rg_status st = trialArgs(ctx, s, n, pos, type, q, molecule, energies, overlap, "rg_trial_energies");
+  if (st != RG_OK || n == 0) return st;
+  RG_ENTER(ctx);
+  Scratch sc{ctx};
+  RG_HIP(ctx, hipMalloc(&sc.p, size_t(n) * kTrialStageBytes));
+  return trialEnergiesStaged(ctx, s, n, pos, type, q, molecule, skip_molecule, energies, overlap, sc.p);
+}
+
+rg_status rg_trial_energies_noalloc(rg_ctx* ctx, const rg_system* s, int n, const double* pos, const int* type,
+                                    const double* q, const int* molecule, int skip_molecule, double* energies,
+                                    int* overlap)
+{
+  rg_status st = trialArgs(ctx, s, n, pos, type, q, molecule, energies, overlap, "rg_trial_energies_noalloc");
+  if (st != RG_OK) return st;
+  if (n > RG_TRIAL_NOALLOC_MAX_SITES)
+    return fail(ctx, RG_ERR_INVALID, "rg_trial_energies_noalloc: %d sites exceed the staging array's %d", n,
+                RG_TRIAL_NOALLOC_MAX_SITES);
+  if (n == 0) return RG_OK;
+  RG_ENTER(ctx);
+  void* stage = nullptr;
+  st = scratchAddress(ctx, &stage);
+  if (st != RG_OK) return st;
+  return trialEnergiesStaged(ctx, s, n, pos, type, q, molecule, skip_molecule, energies, overlap, stage);
 }
 
 rg_status rg_rosenbluth_select(rg_ctx* ctx, int n_sets, int k, const double* energies, const int* overlap,
