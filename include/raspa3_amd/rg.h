@@ -152,6 +152,24 @@ rg_status rg_grid_build_many_async(rg_ctx* ctx, const rg_forcefield* ff, const r
                                    const rg_grid_type* types, const int* probes, const double* cutoffs,
                                    rg_grid* const* grids);
 
+/* Arithmetic of the Ewald real-space pair terms of a tricubic grid build.  The reference computes every pair term in
+ * FP64 (src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56). */
+typedef enum rg_grid_precision
+{
+  RG_GRID_FP64 = 0,      /* every pair term in FP64 (what rg_grid_build_many does) */
+  RG_GRID_SPLIT_FP32 = 1 /* framework images at least split_radius from every point of a 64-point block: radial
+                          * factors in FP32 (hardware rsq and exp2, a fitted FP32 erfcx polynomial), accumulated in
+                          * FP64; nearer images in FP64.  Lennard-Jones grids and triquintic grids stay FP64. */
+} rg_grid_precision;
+
+/* System::createFrameworkInterpolationGrids (src/raspakit/system_interpolation_grids.cpp:176-238) as
+ * rg_grid_build_many, with the precision of the Ewald real-space grid chosen; async != 0 returns after the launch.
+ * RG_ERR_UNSUPPORTED when the FP32 erfcx polynomial cannot reach 4e-8 relative on [alpha split_radius, alpha rc]. */
+rg_status rg_grid_build_many_precision(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, int n_grids,
+                                       const rg_grid_type* types, const int* probes, const double* cutoffs,
+                                       rg_grid* const* grids, rg_grid_precision precision, double split_radius,
+                                       int async);
+
 /* Copies 'count' doubles starting at 'offset' (in doubles) between the grid and host memory. */
 rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size_t count, double* host);
 rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count, const double* host);
@@ -369,6 +387,44 @@ rg_status rg_molecule_weights(rg_ctx* ctx, const rg_system* sys, const rg_compon
                               double* log_weight);
 /* Adsorbate energy of every system recomputed from scratch (System::computeTotalEnergies, src/raspakit/system.cpp:690). */
 rg_status rg_mc_total_energy(rg_ctx* ctx, const rg_mc* mc, double* out);
+
+/* ---------------------------------------------------------------------------------------------
+ * Statistics of a run, as the reference reports them (host-side, no device work).  Blocks are the reference's 5 block
+ * averages (rg_mc_state_t carries their weighted sums); a confidence interval is the 95 % half-width
+ * t_(n-1) s / sqrt(n) over the blocks whose weight exceeds half the first block's, 0 with fewer than 3 such blocks.
+ * ------------------------------------------------------------------------------------------- */
+
+/* blockErrorEstimate (src/raspakit/averages.ixx:144-169; Student-t deviates :28-34, 95 % :46): the confidence
+ * interval of n <= 21 block averages about 'average'. */
+rg_status rg_block_error(int n, const double* block_averages, double average, double* ci);
+
+/* BlockAverage<double>::average (src/raspakit/properties/property_block_average.ixx:108-150) of a directly averaged
+ * property: mean = sum(sums) / sum(weights), ci over the block averages sums[b] / weights[b]. */
+rg_status rg_block_average(int n_blocks, const double* sums, const double* weights, double* mean, double* ci);
+
+/* EnthalpyOfAdsorptionTerms (src/raspakit/properties/enthalpy.ixx:101-200) block-averaged as a composite property
+ * (property_block_average.ixx:108-126): per block, from the weighted sums of U, N_i, U N_i and N_i N_j (row-major
+ * [block][i] and [block][i][j], as in rg_mc_state_t), Delta H_i = sum_j (<U N_j> - <U><N_j>) (M^-1)_ji - k_B T with
+ * M_ij = <N_i N_j> - <N_i><N_j>; mean[i] = the mean over the blocks, ci[i] its interval.  Internal energy units
+ * (the isosteric heat is -Delta H); temperature in K.  A singular block matrix gives NaN. */
+rg_status rg_enthalpy_of_adsorption(int n_blocks, int n_components, const double* weight, const double* sum_u,
+                                    const double* sum_n, const double* sum_un, const double* sum_nn,
+                                    double temperature, double* mean, double* ci);
+
+/* PropertyWidom::result / chemicalPotentialResult / fugacityResult (src/raspakit/properties/widom.ixx:176-230):
+ * from the per-block weighted sums of the Widom Rosenbluth weight W and of the number density N/V (A^-3):
+ * mu_excess = -kT ln<W>, mu_ideal_gas = kT ln<N/V>, mu_total = their sum, fugacity = kT exp(mu_total / kT), in
+ * internal energy units (fugacity in internal energy / A^3); each with its block interval. */
+typedef struct rg_widom_stats_t
+{
+  double rosenbluth_weight, rosenbluth_weight_ci;
+  double mu_excess, mu_excess_ci;
+  double mu_ideal_gas, mu_ideal_gas_ci;
+  double mu_total, mu_total_ci;
+  double fugacity, fugacity_ci;
+} rg_widom_stats_t;
+rg_status rg_widom_statistics(int n_blocks, const double* weight, const double* sum_w, const double* sum_density,
+                              double temperature, rg_widom_stats_t* out);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned first-bead proposal.  The reference has no counterpart: its first-bead trials are uniform in the cell

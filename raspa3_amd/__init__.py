@@ -289,13 +289,20 @@ def rosenbluth_select(ctx, energies, overlap, beta, uniforms):
     return sel, lw
 
 
-def build_grids(ctx, ff, fw, specs, sync=True):
-    """Builds several equal-dimension grids in one fused pass.  specs: list of (Grid, type, probe, cutoff)."""
+def build_grids(ctx, ff, fw, specs, sync=True, split_radius=None):
+    """Builds several equal-dimension grids in one fused pass.  specs: list of (Grid, type, probe, cutoff).
+    split_radius: None for FP64 pair terms throughout, else the radius beyond which the Ewald real-space pair terms are
+    computed in FP32 (rg_grid_build_many_precision)."""
     n = len(specs)
     types = L.i32([s[1] for s in specs])
     probes = L.i32([s[2] for s in specs])
     cuts = L.f64([s[3] for s in specs])
     arr = (C.c_void_p * n)(*[s[0].h.value for s in specs])
+    if split_radius is not None:
+        L.check(lib().rg_grid_build_many_precision(ctx.h, ff.h, fw.h, n, L.i(types), L.i(probes), L.d(cuts), arr,
+                                                   L.RG_GRID_SPLIT_FP32, float(split_radius), int(not sync)),
+                ctx.h, "rg_grid_build_many_precision")
+        return
     fn = lib().rg_grid_build_many if sync else lib().rg_grid_build_many_async
     L.check(fn(ctx.h, ff.h, fw.h, n, L.i(types), L.i(probes), L.d(cuts), arr), ctx.h, "rg_grid_build_many")
 
@@ -323,6 +330,58 @@ def grid_check(ctx, ff, fw, lj_grid, probe, coulomb_grid=None, probe_charge=0.0,
 
     return dict(n_points=out.n_points, lennard_jones=stats(out.lennard_jones), n_coulomb_points=out.n_coulomb_points,
                 coulomb=stats(out.coulomb))
+
+
+def block_error(block_averages, average):
+    """rg_block_error: 95 % confidence interval of block averages (the reference's blockErrorEstimate)."""
+    b = L.f64(block_averages)
+    ci = C.c_double()
+    L.check(lib().rg_block_error(len(b), L.d(b), float(average), C.byref(ci)), None, "rg_block_error")
+    return ci.value
+
+
+def block_average(sums, weights):
+    """rg_block_average: (mean, ci) of a directly averaged property from per-block weighted sums and weights."""
+    s, w = L.f64(sums), L.f64(weights)
+    mean, ci = C.c_double(), C.c_double()
+    L.check(lib().rg_block_average(len(s), L.d(s), L.d(w), C.byref(mean), C.byref(ci)), None, "rg_block_average")
+    return mean.value, ci.value
+
+
+def enthalpy_of_adsorption(weight, sum_u, sum_n, sum_un, sum_nn, temperature):
+    """rg_enthalpy_of_adsorption: per-component enthalpy of adsorption (internal units; the isosteric heat is minus
+    it) and its 95 % block interval, from per-block sums shaped [b], [b], [b, c], [b, c], [b, c, c]."""
+    w = L.f64(weight)
+    nb = len(w)
+    n = L.f64(sum_n).reshape(nb, -1)
+    nc = n.shape[1]
+    mean, ci = np.zeros(nc), np.zeros(nc)
+    L.check(lib().rg_enthalpy_of_adsorption(nb, nc, L.d(w), L.d(L.f64(sum_u)), L.d(n), L.d(L.f64(sum_un)),
+                                            L.d(L.f64(sum_nn)), float(temperature), L.d(mean), L.d(ci)), None,
+            "rg_enthalpy_of_adsorption")
+    return mean, ci
+
+
+def widom_statistics(weight, sum_w, sum_density, temperature):
+    """rg_widom_statistics: Rosenbluth weight, excess / ideal-gas / total chemical potential and fugacity with their
+    95 % block intervals (internal units)."""
+    w = L.f64(weight)
+    out = L.WidomStats()
+    L.check(lib().rg_widom_statistics(len(w), L.d(w), L.d(L.f64(sum_w)), L.d(L.f64(sum_density)), float(temperature),
+                                      C.byref(out)), None, "rg_widom_statistics")
+    return {n: getattr(out, n) for n, _ in L.WidomStats._fields_}
+
+
+def mc_statistics(state, n_components, temperature):
+    """Loading per component (mean and 95 % block interval) and enthalpy of adsorption per component of one system
+    from its rg_mc_state_t block sums, as the reference reports them."""
+    w = state["block_count"]
+    nc = n_components
+    loading = [block_average(state["block_n"][:, c], w) for c in range(nc)]
+    dh, dh_ci = enthalpy_of_adsorption(w, state["block_energy"], state["block_n"][:, :nc],
+                                       state["block_energy_n"][:, :nc], state["block_nn"][:, :nc, :nc], temperature)
+    return dict(loading=np.array([m for m, _ in loading]), loading_ci=np.array([c for _, c in loading]),
+                enthalpy=dh, enthalpy_ci=dh_ci)
 
 
 def component_desc(d):
