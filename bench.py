@@ -109,28 +109,66 @@ def incutoff_counts(fw, lj_types, rc, nsample=20000, seed=0):
     return n_all.mean(), n_lj.mean()
 
 
-def cpu_baseline(fw, spec, dims, target_s=12.0):
-    """Times the FP64 oracle on the host cores over whole z-planes of the three grids."""
+def cpu_baseline(fw, spec, dims, target_s=12.0, gpu_grids=None):
+    """Times the FP64 oracle (the reference's loop order and arithmetic) on every host core over whole z-planes of the
+    three grids: median of 3 runs of the baseline-ISA build (the reference forbids -march) and of a -march=native
+    build.  With gpu_grids ({name: Grid}) the oracle's planes of the native run are also compared with the same planes
+    of the GPU grids: max and RMS |gpu - oracle| per stored channel in K."""
     import oracle
 
     pairs = oracle.pairs_from_spec(spec)
     cores = host_cores()
-    grids = [(0, spec.index("C_co2")), (0, spec.index("O_co2")), (3, 0)]
+    grids = [("C_co2", 0, spec.index("C_co2")), ("O_co2", 0, spec.index("O_co2")), ("ewald", 3, 0)]
 
-    def run(planes):
+    def run(planes, native):
         t = time.perf_counter()
-        for gt, probe in grids:
-            oracle.grid_build(fw.cell, fw.rect, dims, gt, fw.pos, fw.type, fw.charge, pairs, probe,
-                              spec.rc_framework, spec.rc_framework, spec.alpha, k_range=(0, planes), nthreads=cores)
-        return time.perf_counter() - t
+        out = {}
+        for name, gt, probe in grids:
+            out[name] = oracle.grid_build(fw.cell, fw.rect, dims, gt, fw.pos, fw.type, fw.charge, pairs, probe,
+                                          spec.rc_framework, spec.rc_framework, spec.alpha, k_range=(0, planes),
+                                          nthreads=cores, native=native)
+        return time.perf_counter() - t, out
 
-    t1 = run(1)
-    planes = int(max(1, min(dims[2], round(target_s / max(t1, 1e-3)))))
-    t = run(planes) if planes > 1 else t1
+    t1, _ = run(1, False)
+    planes = int(max(1, min(dims[2], round(target_s / 6.0 / max(t1, 1e-3)))))
     pts = 3 * planes * dims[0] * dims[1]
-    return dict(value=pts / t, unit="grid points/s", cores=cores, kind="port",
+    rates, ref = {}, None
+    for native in (False, True):
+        ts = []
+        for _ in range(3):
+            t, out = run(planes, native)
+            ts.append(t)
+            if native and ref is None:
+                ref = out
+        rates["native" if native else "baseline_isa"] = dict(median_s=float(np.median(ts)), runs_s=ts,
+                                                              points_per_s=pts / float(np.median(ts)))
+    value = rates["baseline_isa"]["points_per_s"]
+    base = dict(value=value, unit="grid points/s", cores=cores, kind="port", builds=rates,
                 sample=f"FP64 oracle (reference loop order, all framework atoms per point), {planes} z-plane(s) x "
-                       f"{dims[0]}x{dims[1]} points x 3 grids = {pts} points in {t:.2f} s on {cores} threads")
+                       f"{dims[0]}x{dims[1]} points x 3 grids = {pts} points per run, median of 3 runs on {cores} "
+                       f"threads (rows of points split over the threads); value = the baseline-ISA build, "
+                       f"-march=native beside it")
+    if gpu_grids:
+        k2e = oracle.units()["kelvin_to_energy"]
+        err = {}
+        for name, g in gpu_grids.items():
+            r = ref["ewald" if name.startswith("ewald") else name].reshape(-1, 8)
+            got = g.download(0, r.size).reshape(-1, 8)
+            d = np.abs(got - r) / k2e
+            err[name] = dict(max_abs_K=d.max(0).tolist(), rms_abs_K=np.sqrt((d * d).mean(0)).tolist())
+        base["grid_error_vs_oracle"] = dict(points_per_grid=planes * dims[0] * dims[1], planes=[0, planes], **err)
+    return base
+
+
+def grid_channel_errors(a, b, chunk_points, n_points, k_to_e):
+    """Max and RMS |a - b| per stored channel (K) of two tricubic grids, streamed through host memory in chunks."""
+    mx, s2 = np.zeros(8), np.zeros(8)
+    for p0 in range(0, n_points, chunk_points):
+        c = min(chunk_points, n_points - p0)
+        d = np.abs(a.download(8 * p0, 8 * c) - b.download(8 * p0, 8 * c)).reshape(-1, 8) / k_to_e
+        mx = np.maximum(mx, d.max(0))
+        s2 += (d * d).sum(0)
+    return mx, np.sqrt(s2 / n_points)
 
 
 def bench_grids(ctx, args):
@@ -146,15 +184,21 @@ def bench_grids(ctx, args):
     ff = ra.ForceField(ctx, spec)
     f = ra.Framework(ctx, fw)
     grids = [ra.Grid(ctx, fw.cell, fw.rect, dims) for _ in range(3)]
+    g64 = ra.Grid(ctx, fw.cell, fw.rect, dims)  # the same Ewald grid with every pair term in FP64
     specs = [(grids[0], RG_GRID_LENNARD_JONES, spec.index("C_co2"), spec.rc_framework),
              (grids[1], RG_GRID_LENNARD_JONES, spec.index("O_co2"), spec.rc_framework),
              (grids[2], RG_GRID_EWALD_REAL, 0, spec.rc_coulomb)]
-    log(f"grids: 3 x {dims} ({3 * npts} points), {fw.n} framework sites")
-    # one step = the two LJ grids (one launch) and the Ewald real-space grid (one launch), each call bracketed by HIP
-    # events on the library's stream, so the per-kernel averages come from the timed region itself
+    split = args.split_radius
+    log(f"grids: 3 x {dims} ({3 * npts} points), {fw.n} framework sites, Ewald pair terms in FP32 beyond {split} A")
+    # one step = the two LJ grids (one launch, FP64) and the Ewald real-space grid (one launch, FP32 pair terms
+    # beyond the split radius, FP64 within it and for every accumulation), each call bracketed by HIP events on the
+    # library's stream, so the per-kernel averages come from the timed region itself
     lj_specs, q_specs = specs[:2], specs[2:]
+    q64_specs = [(g64, RG_GRID_EWALD_REAL, 0, spec.rc_coulomb)]
     for _ in range(max(1, args.warmup)):
-        ra.build_grids(ctx, ff, f, specs)
+        ra.build_grids(ctx, ff, f, lj_specs)
+        ra.build_grids(ctx, ff, f, q_specs, split_radius=split)
+        ra.build_grids(ctx, ff, f, q64_specs)
     stream = torch.cuda.ExternalStream(ctx.stream, device=torch.device("cuda", 0))
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -164,7 +208,7 @@ def bench_grids(ctx, args):
         ev[k][0].record(stream)
         ra.build_grids(ctx, ff, f, lj_specs, sync=False)
         ev[k][1].record(stream)
-        ra.build_grids(ctx, ff, f, q_specs, sync=False)
+        ra.build_grids(ctx, ff, f, q_specs, sync=False, split_radius=split)
         ev[k][2].record(stream)
     ctx.sync()
     torch.cuda.synchronize()
@@ -174,26 +218,57 @@ def bench_grids(ctx, args):
     q_ms = sum(e[1].elapsed_time(e[2]) for e in ev) / args.steps
     value = 3 * npts / (ms_per_step / 1000.0)
     log(f"grids: ms/step {ms_per_step:.3f} (LJ kernel {lj_ms:.3f}, Ewald kernel {q_ms:.3f}); {value:.4e} grid points/s")
+    # the FP64 Ewald kernel, timed the same way outside the headline step
+    ev64 = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    ctx.sync()
+    for k in range(args.steps):
+        ev64[k].record(stream)
+        ra.build_grids(ctx, ff, f, q64_specs, sync=False)
+    ev64[args.steps].record(stream)
+    ctx.sync()
+    q64_ms = sum(ev64[k].elapsed_time(ev64[k + 1]) for k in range(args.steps)) / args.steps
+    log(f"grids: FP64 Ewald kernel {q64_ms:.3f} ms")
 
     n_all, n_lj = incutoff_counts(fw, [spec.index("O")], spec.rc_framework)
     flop_q = npts * FLOP_EWALD_TRIC * n_all
     flop_lj = npts * 2 * FLOP_LJ_TRIC * n_lj
     achieved = flop_q / (q_ms / 1000.0) / 1e12
     lj_achieved = flop_lj / (lj_ms / 1000.0) / 1e12
+    a64 = flop_q / (q64_ms / 1000.0) / 1e12
     roof = dict(bound="valu", achieved=round(achieved, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                 frac=round(achieved / FP64_PEAK_TFLOPS, 4), traffic=EWALD_GRID_HBM_BYTES,
                 traffic_algorithmic=float(npts * 64), traffic_note="HBM bytes per Ewald launch (rocprofv3 "
                 "FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy); the kernel writes 64 B per point and is bound "
-                "by FP64 VALU, not HBM",
-                kernel="k_grid_build<0,true,true> (Ewald real-space grid, triclinic), the dominant kernel of the step",
+                "by VALU issue, not HBM",
+                peak_note="FP64 vector peak; the kernel's FP32 instructions are unpacked (packed FP32 is compiled "
+                          "out), and unpacked FP32 issues at the FP64 rate, so 78.6 TFLOP/s bounds both; against "
+                          "the packed-FP32 figure of 157.3 TFLOP/s the fraction halves",
+                frac_vs_packed_fp32_peak=round(achieved / 157.3, 4),
+                kernel=f"k_grid_build<0,true,true,true> (Ewald real-space grid, triclinic, FP32 pair terms beyond "
+                       f"{split} A), the dominant kernel of the step",
                 flop_per_launch=float(flop_q), kernel_ms=round(q_ms, 4),
                 incutoff_pairs_per_point=dict(lj_active=round(n_lj, 2), charged=round(n_all, 2)),
-                other_kernels=dict(lj=dict(kernel="k_grid_build<2,false,true> (C_co2 + O_co2 LJ grids)",
+                other_kernels=dict(lj=dict(kernel="k_grid_build<2,false,true> (C_co2 + O_co2 LJ grids, FP64)",
                                            kernel_ms=round(lj_ms, 4), achieved=round(lj_achieved, 3),
                                            frac=round(lj_achieved / FP64_PEAK_TFLOPS, 4),
-                                           flop_per_launch=float(flop_lj))),
+                                           flop_per_launch=float(flop_lj)),
+                                   ewald_fp64=dict(kernel="k_grid_build<0,true,true> (Ewald grid, every pair term "
+                                                          "FP64)", kernel_ms=round(q64_ms, 4), achieved=round(a64, 3),
+                                                   frac=round(a64 / FP64_PEAK_TFLOPS, 4))),
                 step_achieved=round((flop_q + flop_lj) / ((lj_ms + q_ms) / 1000.0) / 1e12, 3))
-    state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, dims=dims, pairs=(n_all, n_lj))
+    # per-point error of the split Ewald grid against the FP64 one, every point of the grid
+    import oracle
+    k2e = oracle.units()["kelvin_to_energy"]
+    t = time.perf_counter()
+    mx, rms = grid_channel_errors(grids[2], g64, dims[0] * dims[1] * 25, npts, k2e)
+    split_err = dict(split_radius_A=split, points=npts, max_abs_K=mx.tolist(), rms_abs_K=rms.tolist(),
+                     fp64_kernel_ms=round(q64_ms, 4), split_kernel_ms=round(q_ms, 4),
+                     speedup=round(q64_ms / q_ms, 4), seconds=round(time.perf_counter() - t, 2),
+                     channels="E, dE/ds0, dE/ds1, dE/ds2, d2E/ds0ds1, d2E/ds0ds2, d2E/ds1ds2, d3E/ds0ds1ds2 (stored, "
+                              "scaled by the cell counts)")
+    log(f"split vs FP64 Ewald grid, all points: max |d| K {np.array2string(mx, precision=2)}")
+    state = dict(fw=fw, spec=spec, ff=ff, f=f, grids=grids, g64=g64, dims=dims, pairs=(n_all, n_lj),
+                 split_error=split_err)
     # the test the reference runs after building (1e5 random points per LJ grid, with the Ewald grid scaled by the
     # probe's charge); outside the timed region
     check = {}
@@ -766,6 +841,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--split-radius", type=float, default=7.0,
+                    help="Ewald real-space pair terms beyond this distance (A) in FP32 in the headline grid step")
     ap.add_argument("--grids-only", action="store_true", help="measure the headline grid build only")
     ap.add_argument("--gcmc-only", default="", metavar="NAMES",
                     help="measure the headline grid build and only the GCMC comparisons named (itq29, mfi, channels; "
@@ -820,7 +897,7 @@ def main():
     torch.cuda.set_device(0)
     ctx = ra.Context(0)
     value, ms_per_step, npts, roof, g1 = bench_grids(ctx, args)
-    extra = {"grid_check": g1["grid_check"]}
+    extra = {"grid_check": g1["grid_check"], "grid_split_error": g1["split_error"]}
     if args.quintic_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
     if args.gcmc_only:
@@ -840,7 +917,10 @@ def main():
         extra["gcmc_mixture"] = bench_gcmc_mixture(ctx, args)
     base = None
     if not args.no_cpu_baseline:
-        base = cpu_baseline(g1["fw"], g1["spec"], g1["dims"], args.cpu_seconds)
+        base = cpu_baseline(g1["fw"], g1["spec"], g1["dims"], args.cpu_seconds,
+                            gpu_grids={"C_co2": g1["grids"][0], "O_co2": g1["grids"][1],
+                                       f"ewald_split_{args.split_radius:g}A": g1["grids"][2],
+                                       "ewald_fp64": g1["g64"]})
         log(f"cpu baseline {base['value']:.4e} points/s ({base['sample']})")
         if "gcmc" in extra:
             extra["gcmc"]["cpu_baseline"] = cpu_baseline_gcmc()
@@ -851,12 +931,15 @@ def main():
     fw, spec = g1["fw"], g1["spec"]
     out = dict(metric="framework-energy evals/s; GCMC accepted insertions/s at matched loading",
                value=value, unit="grid points/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-               ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f64 (Ewald pair terms beyond the split radius in f32, accumulated in f64)",
                data="synthetic",
                config=dict(workload="CO2 grids with charges: triclinic 40x40x30 A gamma=100, 4095 sites, "
                                     "C_co2 LJ + O_co2 LJ + Ewald-real grids at h=0.1 A (394x394x300 each), "
-                                    "one LJ launch for both LJ grids + one Ewald launch per step", grid_points_per_step=3 * npts, framework_sites=fw.n,
-                           cutoff=spec.rc_framework, alpha=spec.alpha),
+                                    "one LJ launch for both LJ grids (FP64) + one Ewald launch per step (FP32 pair "
+                                    f"terms beyond {args.split_radius:g} A, FP64 within it and for every sum)",
+                           grid_points_per_step=3 * npts, framework_sites=fw.n, cutoff=spec.rc_framework,
+                           alpha=spec.alpha, split_radius_A=args.split_radius),
                roofline=roof, cpu_baseline=base, **extra)
     print(json.dumps(out), flush=True)
     ctx.close()

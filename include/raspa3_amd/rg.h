@@ -305,7 +305,8 @@ typedef struct rg_component_desc
   double position[24];  /* n_atoms * 3, reference geometry (A); unused for flexible chains */
   int type[8];          /* pseudo-atom indices */
   double charge[8];
-  double mol_fraction;  /* Component::molFraction */
+  double mol_fraction;  /* Component::molFraction: the fugacity is mol_fraction x pressure (components are drawn
+                         * uniformly per step, System::randomComponent, src/raspakit/system.ixx:372-375) */
   int flexible;
   double bond_length;       /* A */
   double bend_theta0_deg;
@@ -324,7 +325,8 @@ typedef struct rg_mc_params
   double p_rotation;
   double max_translation;   /* A */
   double max_rotation;      /* rad */
-  int min_steps_per_cycle;  /* steps per cycle = max(this, N) (src/raspakit/drivers/monte_carlo.cpp:301) */
+  int min_steps_per_cycle;  /* steps per cycle = max(this, N) x number of components
+                             * (src/raspakit/drivers/monte_carlo.cpp:301) */
   int max_molecules;        /* per system; each system's capacity_atoms must be >= max_molecules * max(n_atoms) */
   unsigned long long seed;  /* Philox4x32-10 key; trials are addressed by (system, step, slot) */
   long long production_cycles; /* cycles over which the 5 block averages are spread */

@@ -144,12 +144,12 @@ def grid_build(cell, rect, dims, grid_type, fw_pos, fw_type, fw_q, pairs, probe,
     """Full-loop reference builder over z-planes k_range (default all).  Returns array (nz_sel, ny, nx, 8)."""
     nx, ny, nz = dims
     k0, k1 = (0, nz) if k_range is None else k_range
-    out = np.zeros(8 * nx * ny * nz)
+    out = np.zeros(8 * nx * ny * (k1 - k0))
     c9, r, na, p, t, q, nt, e, s, sh, lj = _fw_args(cell, rect, fw_pos, fw_type, fw_q, pairs)
     st = lib(native).orc_grid_build(c9, r, nx, ny, nz, grid_type, na, p, t, q, nt, e, s, sh, lj, probe, rc_vdw,
                                     rc_supercell, alpha, overlap, k0, k1, nthreads, _d(out))
     assert st == 0
-    return out.reshape(nz, ny, nx, 8)[k0:k1]
+    return out.reshape(k1 - k0, ny, nx, 8)
 
 
 def grid_points(cell, rect, dims, grid_type, fw_pos, fw_type, fw_q, pairs, probe, rc_vdw, rc_supercell, alpha, ijk,

@@ -677,10 +677,12 @@ int orc_grid_build(const double* cell9, int rect, int nx, int ny, int nz, int gr
   FF ff{ntypes, pair_eps, pair_sigma, pair_shift, pair_lj};
   const double delta[3] = {1.0 / double(nx - 1), 1.0 / double(ny - 1), 1.0 / double(nz - 1)};
   const int ns = static_cast<int>(stype.size());
-  auto work = [&](int k0, int k1)
+  // rows (j, k) of planes k_begin .. k_end - 1, in the reference's k -> j -> i order within each worker's range
+  auto work = [&](long r0, long r1)
   {
-    for (int k = k0; k < k1; ++k)
-      for (int j = 0; j < ny; ++j)
+    for (long row = r0; row < r1; ++row)
+    {
+      const int k = k_begin + static_cast<int>(row / ny), j = static_cast<int>(row % ny);
         for (int i = 0; i < nx; ++i)
         {
           double s[3] = {double(i) * delta[0], double(j) * delta[1], double(k) * delta[2]}, p[3];
@@ -695,7 +697,8 @@ int orc_grid_build(const double* cell9, int rect, int nx, int ny, int nz, int gr
           v[5] /= double(nx - 1) * double(nz - 1);
           v[6] /= double(ny - 1) * double(nz - 1);
           v[7] /= double(nx - 1) * double(ny - 1) * double(nz - 1);
-          double* o = out + 8 * (std::size_t(i) + std::size_t(nx) * (std::size_t(j) + std::size_t(ny) * k));
+          // out holds planes k_begin .. k_end - 1 only
+          double* o = out + 8 * (std::size_t(i) + std::size_t(nx) * (std::size_t(j) + std::size_t(ny) * (k - k_begin)));
           if (v[0] > overlap)
           {
             o[0] = 2.0 * overlap;
@@ -705,16 +708,17 @@ int orc_grid_build(const double* cell9, int rect, int nx, int ny, int nz, int gr
           else
             for (int l = 0; l < 8; ++l) o[l] = v[l];
         }
+    }
   };
+  const long rows = long(k_end - k_begin) * ny;
   if (nthreads <= 1)
-    work(k_begin, k_end);
+    work(0, rows);
   else
   {
     std::vector<std::thread> th;
-    int total = k_end - k_begin;
     for (int t = 0; t < nthreads; ++t)
     {
-      int a = k_begin + (total * t) / nthreads, b = k_begin + (total * (t + 1)) / nthreads;
+      const long a = (rows * t) / nthreads, b = (rows * (t + 1)) / nthreads;
       if (a < b) th.emplace_back(work, a, b);
     }
     for (auto& x : th) x.join();

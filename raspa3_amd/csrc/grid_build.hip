@@ -54,6 +54,7 @@ constexpr int MAXG = 4;    // LJ grids per launch
 constexpr int MAXT = 16;   // distinct framework pseudo-atom types
 constexpr int CH = 256;    // candidates per LDS chunk
 constexpr int MAXROWS = 1024;
+constexpr int kSplitDeg = 9;  // degree of the FP32 erfcx polynomial of the split-precision Ewald grid
 
 struct GridArgs
 {
@@ -77,6 +78,12 @@ struct GridArgs
   double coulomb;
   double overlap;
   double* out[MAXG + 1];
+  // split precision of the Ewald real-space grid: images at least sqrt(rsplit2) from every point of a wavefront take
+  // FP32 radial factors (erfcx from a polynomial in t = r xs + xo, |t| <= 1)
+  double rsplit2;
+  double a2l2e;  // alpha^2 / ln 2
+  float xs32, xo32, k2a32, alpha2_32;
+  float ecx[kSplitDeg + 1];
 };
 
 // The hardware reciprocal and reciprocal square root are within 5.2e-8 relative (measured over 1e6 arguments in
@@ -358,7 +365,7 @@ __device__ __forceinline__ int stageChunk(const GridArgs& A, const TilePoint& T,
   return ncand;
 }
 
-template <int NLJ, bool COUL, bool TRIC>
+template <int NLJ, bool COUL, bool TRIC, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
 {
   constexpr int NG = NLJ + (COUL ? 1 : 0);
@@ -445,7 +452,37 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
           }
         }
         // a Coulomb-only launch stages charged images only
-        if (COUL && (NLJ == 0 || c1.y != 0.0))
+        if (COUL && SPLIT && !__ballot(rr < A.rsplit2))
+        {
+          // every lane of the wavefront is at least the split radius from this image: the same radial factors in
+          // FP32 (hardware rsq and exp2, erfcx from the fitted polynomial), accumulated in FP64 like the rest.  The
+          // exponent alpha^2 r^2 / ln 2 is reduced in FP64 so that only its fractional part passes through FP32.
+          const float rr32 = static_cast<float>(rr);
+          const float ir = __builtin_amdgcn_rsqf(rr32);
+          const float r = rr32 * ir;
+          const float t = fmaf(r, A.xs32, A.xo32);
+          float px = A.ecx[kSplitDeg];
+#pragma unroll
+          for (int j = kSplitDeg - 1; j >= 0; --j) px = fmaf(px, t, A.ecx[j]);
+          const double y = rr * A.a2l2e;
+          const double fl = floor(y);
+          const float ex = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(fl - y)),
+                                                   -static_cast<int>(fl));
+          const float qex = static_cast<float>(c1.y) * ex;  // q exp(-alpha^2 r^2)
+          const float ec = qex * px;                         // q erfc(alpha r)
+          const float etr = A.k2a32 * r * qex;
+          const float ar2 = A.alpha2_32 * rr32;
+          const float ir2 = ir * ir;
+          const float ir3 = ir2 * ir;
+          const float ir5 = ir3 * ir2;
+          const float ir7 = ir5 * ir2;
+          const float e = ec * ir;
+          const float f1 = -(etr + ec) * ir3;
+          const float f2 = fmaf(etr, fmaf(2.0f, ar2, 3.0f), 3.0f * ec) * ir5;
+          const float f3 = -fmaf(etr, fmaf(ar2, fmaf(4.0f, ar2, 10.0f), 15.0f), 15.0f * ec) * ir7;
+          acc[NG - 1].add(double(e), double(f1), double(f2), double(f3), u0, u1, u2, u01, u02, u12, u012);
+        }
+        else if (COUL && (NLJ == 0 || c1.y != 0.0))
         {
           // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 with a unit probe charge;
           // the staged charge already carries the Coulomb conversion factor.
@@ -664,11 +701,13 @@ KernelFn pick2(bool tric)
   return tric ? k_grid_build<NLJ, COUL, true> : k_grid_build<NLJ, COUL, false>;
 }
 
-KernelFn pick(int nlj, bool coul, bool tric)
+KernelFn pick(int nlj, bool coul, bool tric, bool split)
 {
   switch (nlj)
   {
-    case 0: return coul ? pick2<0, true>(tric) : nullptr;
+    case 0:
+      if (coul && split) return tric ? k_grid_build<0, true, true, true> : k_grid_build<0, true, false, true>;
+      return coul ? pick2<0, true>(tric) : nullptr;
     case 1: return coul ? pick2<1, true>(tric) : pick2<1, false>(tric);
     case 2: return coul ? pick2<2, true>(tric) : pick2<2, false>(tric);
     case 3: return coul ? pick2<3, true>(tric) : pick2<3, false>(tric);
@@ -792,9 +831,57 @@ LJPar ljPar(const rg_forcefield* ff, int probe, int type)
   return P;
 }
 
+// erfcx(x) = exp(x^2) erfc(x) on [x0, x1] as a polynomial of degree kSplitDeg in t = x * xs + xo (|t| <= 1): the
+// Chebyshev interpolant at kSplitDeg + 1 Chebyshev points, in monomial form, with FP32 coefficients.  Returns the
+// largest relative error of the FP32-coefficient polynomial (evaluated in FP64) against the FP64 library erfc over
+// 4001 points of the interval.
+double fitErfcxF32(double x0, double x1, float* coef, double& xs, double& xo)
+{
+  constexpr int N = kSplitDeg + 1;
+  const double mid = 0.5 * (x0 + x1), half = 0.5 * (x1 - x0);
+  xs = 1.0 / half;
+  xo = -mid / half;
+  auto erfcx = [](double x) { return std::exp(x * x) * std::erfc(x); };
+  double fval[N], cheb[N];
+  for (int k = 0; k < N; ++k) fval[k] = erfcx(mid + half * std::cos(M_PI * (k + 0.5) / N));
+  for (int j = 0; j < N; ++j)
+  {
+    double sum = 0.0;
+    for (int k = 0; k < N; ++k) sum += fval[k] * std::cos(M_PI * j * (k + 0.5) / N);
+    cheb[j] = (j == 0 ? 1.0 : 2.0) * sum / N;
+  }
+  // T_j(t) in monomials by T_{j+1} = 2 t T_j - T_{j-1}
+  double mono[N] = {0.0}, tprev[N] = {0.0}, tcur[N] = {0.0}, tnext[N];
+  tprev[0] = 1.0;
+  tcur[1] = 1.0;
+  mono[0] += cheb[0];
+  for (int i = 0; i < N; ++i) mono[i] += cheb[1] * tcur[i];
+  for (int j = 2; j < N; ++j)
+  {
+    for (int i = 0; i < N; ++i) tnext[i] = (i > 0 ? 2.0 * tcur[i - 1] : 0.0) - tprev[i];
+    for (int i = 0; i < N; ++i)
+    {
+      mono[i] += cheb[j] * tnext[i];
+      tprev[i] = tcur[i];
+      tcur[i] = tnext[i];
+    }
+  }
+  for (int i = 0; i < N; ++i) coef[i] = static_cast<float>(mono[i]);
+  double worst = 0.0;
+  for (int k = 0; k <= 4000; ++k)
+  {
+    const double x = x0 + (x1 - x0) * k / 4000.0;
+    const double t = x * xs + xo;
+    double p = coef[N - 1];
+    for (int i = N - 2; i >= 0; --i) p = p * t + coef[i];
+    worst = std::max(worst, std::fabs(p / erfcx(x) - 1.0));
+  }
+  return worst;
+}
+
 rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fwc, int n_grids,
                         const rg_grid_type* types, const int* probes, const double* cutoffs, rg_grid* const* grids,
-                        bool sync)
+                        bool sync, int precision = RG_GRID_FP64, double splitRadius = 0.0)
 {
   if (!ctx || !ff || !fwc || n_grids <= 0 || !types || !probes || !cutoffs || !grids)
     return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: invalid arguments");
@@ -904,6 +991,24 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
   A.k2a = 2.0 * ff->alpha / std::sqrt(M_PI);
   A.coulomb = rg::units().coulomb;
   A.overlap = ff->overlap;
+  const bool split = precision == RG_GRID_SPLIT_FP32 && !coul.empty() && splitRadius < rc;
+  if (split)
+  {
+    // images within the split radius of a wavefront keep the FP64 path; the others have alpha r in
+    // [alpha rsplit, alpha rc] (with a relative margin of 1e-4 for the FP32 rounding of r)
+    double xs = 0.0, xo = 0.0;
+    const double err = fitErfcxF32(ff->alpha * splitRadius * (1.0 - 1e-4), ff->alpha * rc * (1.0 + 1e-4), A.ecx, xs, xo);
+    if (!(err < 4e-8))
+      return rg::fail(ctx, RG_ERR_UNSUPPORTED,
+                      "rg_grid_build: the FP32 erfcx polynomial reaches only %.2e relative on alpha r in [%g, %g]; "
+                      "choose a larger split radius", err, ff->alpha * splitRadius, ff->alpha * rc);
+    A.rsplit2 = splitRadius * splitRadius;
+    A.a2l2e = ff->alpha * ff->alpha / M_LN2;
+    A.xs32 = static_cast<float>(ff->alpha * xs);
+    A.xo32 = static_cast<float>(xo);
+    A.k2a32 = static_cast<float>(A.k2a);
+    A.alpha2_32 = static_cast<float>(A.alpha2);
+  }
 
   const bool tric = !fw->rect;
   if (g0->values == 27)
@@ -987,7 +1092,7 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
     const Launch& L = launches[l];
     for (int q = 0; q <= MAXG; ++q) A.out[q] = L.out[q];
     A.ljpar = static_cast<const LJPar*>(ctx->d_table) + l * MAXG * MAXT;
-    KernelFn fn = pick(L.nlj, L.coul, tric);
+    KernelFn fn = pick(L.nlj, L.coul, tric, split);
     if (!fn) return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: nothing to build");
     hipLaunchKernelGGL(fn, dim3(A.ntiles), dim3(256), 0, ctx->stream, A);
     RG_HIP(ctx, hipGetLastError());
@@ -1367,6 +1472,18 @@ rg_status rg_grid_build_many_async(rg_ctx* ctx, const rg_forcefield* ff, const r
                                    rg_grid* const* grids)
 {
   return gridBuildMany(ctx, ff, fw, n_grids, types, probes, cutoffs, grids, false);
+}
+
+rg_status rg_grid_build_many_precision(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, int n_grids,
+                                       const rg_grid_type* types, const int* probes, const double* cutoffs,
+                                       rg_grid* const* grids, rg_grid_precision precision, double split_radius,
+                                       int async)
+{
+  if (precision != RG_GRID_FP64 && precision != RG_GRID_SPLIT_FP32)
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build_many_precision: unknown precision %d", int(precision));
+  if (precision == RG_GRID_SPLIT_FP32 && !(split_radius > 0.0))
+    return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build_many_precision: the split radius must be positive");
+  return gridBuildMany(ctx, ff, fw, n_grids, types, probes, cutoffs, grids, async == 0, int(precision), split_radius);
 }
 
 rg_status rg_grid_check(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, const rg_grid* lj_grid, int probe,

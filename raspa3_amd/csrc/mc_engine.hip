@@ -1196,6 +1196,26 @@ __global__ __launch_bounds__(256) void k_molecule_weights(const SysDev* __restri
   if (active && o == 0) logW[i] = (m == -INFINITY) ? -INFINITY : m + log(sum / M);
 }
 
+// Index of the kth molecule (in slot order) of component ci among the first nmol, or -1: 64 slots per ballot.
+__device__ int kthMoleculeOfComponent(const McSys& Y, int nmol, int ci, int kth, int lane)
+{
+  int seen = 0;
+  for (int base = 0; base < nmol; base += 64)
+  {
+    const int q = base + lane;
+    const unsigned long long b = __ballot(q < nmol && Y.comp[q] == ci);
+    const int cnt = __popcll(b);
+    if (kth < seen + cnt)
+    {
+      unsigned long long bb = b;
+      for (int r = 0; r < kth - seen; ++r) bb &= bb - 1;
+      return base + __ffsll((long long)bb) - 1;
+    }
+    seen += cnt;
+  }
+  return -1;
+}
+
 template <bool CHAIN>
 __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W,
                          const Team& T)
@@ -1205,29 +1225,20 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
   double uMove, uComp, uMol, uAcc;
   uni(R, sys, step, SLOT_MOVE, uMove, uComp);
   uni(R, sys, step, SLOT_PICK, uMol, uAcc);
-  int ci = R.ncomp - 1;
-  {
-    double cum = 0.0;
-    for (int q = 0; q < R.ncomp; ++q)
-    {
-      cum += R.comp[q].molFraction;
-      if (uComp < cum)
-      {
-        ci = q;
-        break;
-      }
-    }
-  }
+  // the component is drawn uniformly (System::randomComponent, src/raspakit/system.ixx:372-375); the molecule that a
+  // translation, rotation or deletion acts on is a random molecule of that component
+  const int ci = min(int(uComp * R.ncomp), R.ncomp - 1);
   const Comp& c = R.comp[ci];
   const double pTrans = R.pTranslation, pRot = pTrans + R.pRotation;
   if (uMove < pRot)
   {
-    // translation or rotation of a random molecule (any component)
+    // translation or rotation of a random molecule of the component
     const bool isTrans = uMove < pTrans;
     L.trials[isTrans ? 2 : 3] += 1;
-    if (L.nmol == 0) return;
-    const int m = min(int(uMol * L.nmol), L.nmol - 1);
-    const Comp& cm = R.comp[Y.comp[m]];
+    if (L.ncomp[ci] == 0) return;
+    const int m = kthMoleculeOfComponent(Y, L.nmol, ci, min(int(uMol * L.ncomp[ci]), L.ncomp[ci] - 1), lane);
+    if (m < 0) return;
+    const Comp& cm = c;
     double d0, d1, d2, d3;
     uni(R, sys, step, SLOT_DISP, d0, d1);
     uni(R, sys, step, SLOT_DISP + 1, d2, d3);
@@ -1347,22 +1358,7 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
   else
   {
     if (nc == 0) return;
-    const int kth = min(int(uMol * nc), nc - 1);
-    int seen = 0;
-    for (int base = 0; base < L.nmol && m < 0; base += 64)
-    {
-      const int q = base + lane;
-      const bool is = q < L.nmol && Y.comp[q] == ci;
-      const unsigned long long b = __ballot(is);
-      const int cnt = __popcll(b);
-      if (kth < seen + cnt)
-      {
-        unsigned long long bb = b;
-        for (int r = 0; r < kth - seen; ++r) bb &= bb - 1;
-        m = base + __ffsll((long long)bb) - 1;
-      }
-      seen += cnt;
-    }
+    m = kthMoleculeOfComponent(Y, L.nmol, ci, min(int(uMol * nc), nc - 1), lane);
     if (m < 0) return;
   }
   double logW, Uext;
@@ -1473,8 +1469,8 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
   }
   for (int cyc = 0; cyc < cycles; ++cyc)
   {
-    // steps per cycle = max(20, N) (src/raspakit/drivers/monte_carlo.cpp:301)
-    const int nsteps = max(R.minStepsPerCycle, L.nmol);
+    // steps per cycle = max(20, N) times the number of components (src/raspakit/drivers/monte_carlo.cpp:301)
+    const int nsteps = max(R.minStepsPerCycle, L.nmol) * R.ncomp;
     for (int s = 0; s < nsteps; ++s)
     {
       gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w], T);

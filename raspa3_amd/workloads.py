@@ -207,8 +207,11 @@ def co2_component(spec, mol_fraction=1.0):
 
 
 def n2_component(spec, mol_fraction=1.0):
-    """N2 of configuration 4: two LJ sites (36 K, 3.31 A, -0.482 e) 0.55 A either side of a +0.964 e centre
-    (examples/basic/2_mc_co2_n2_in_two_independent_boxes/N2.json).  First bead = atom 0, an N site: the reference's
+    """N2 of the CO2/N2 mixture workload: two LJ sites (36 K, 3.31 A, -0.482 e) 0.55 A either side of a +0.964 e
+    centre.  The parameters are the TraPPE N2 model (Potoff and Siepmann, AIChE J. 47 (2001) 1676) that the workload
+    prescribes; the geometry is that of examples/basic/2_mc_co2_n2_in_two_independent_boxes/N2.json, whose force field
+    uses the Martin-Calvo parameters instead (38.298 K, 3.306 A, -0.405 / +0.810 e), so the mixture results are for
+    the TraPPE model, not that example's.  First bead = atom 0, an N site: the reference's
     default CBMC starting bead (src/raspakit/component.ixx:195).  Growing from the charge-only centre instead is
     exact too but samples badly: a centre trial beside a framework O scores about -87,000 K on its own (the N sites
     that keep the whole molecule away are placed only in the orientation step), so a few first-bead weights of

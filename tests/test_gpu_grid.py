@@ -41,6 +41,9 @@ def test_config0_full_grid_matches_oracle(gpu_ctx):
     got = g.download().reshape(dims[2], dims[1], dims[0], 8)
     ref = oracle.grid_build(fw.cell, fw.rect, dims, 0, fw.pos, fw.type, fw.charge, oracle.pairs_from_spec(spec),
                             spec.index("CH4"), 12.0, 12.0, spec.alpha, nthreads=NTHREADS)
+    m, r = channel_errors(got, ref)
+    print(f"config0 CH4 grid vs oracle, every point: max |d| K per channel {np.array2string(m, precision=2)}, "
+          f"rms {np.array2string(r, precision=2)}", flush=True)
     assert_close(got, ref, "config0")
     # overlap clamps are present and reproduced exactly
     clamped = ref[..., 0] == 2.0 * spec.overlap
@@ -87,7 +90,57 @@ def test_config1_fused_grids_match_oracle(gpu_ctx):
     for (g, gt, probe, _), name in zip(specs, ["C_co2", "O_co2", "ewald"]):
         ref = oracle.grid_points(fw.cell, fw.rect, dims, gt, fw.pos, fw.type, fw.charge, pairs, probe, 12.0, 12.0,
                                  spec.alpha, ijk, nthreads=NTHREADS)
-        assert_close(gather(g, dims, ijk), ref, f"config1 {name}")
+        got = gather(g, dims, ijk)
+        m, r = channel_errors(got, ref)
+        print(f"config1 {name} vs oracle, 2008 points: max |d| K per channel {np.array2string(m, precision=2)}, "
+              f"rms {np.array2string(r, precision=2)}", flush=True)
+        assert_close(got, ref, f"config1 {name}")
+
+
+def channel_errors(gpu, ref):
+    """Max and RMS |gpu - ref| per stored channel, in K."""
+    d = np.abs(np.asarray(gpu) - np.asarray(ref)).reshape(-1, 8) / K_TO_E
+    return d.max(0), np.sqrt((d * d).mean(0))
+
+
+SPLIT_RADIUS = 7.0
+
+
+def test_config1_split_precision_ewald_grid(gpu_ctx):
+    """The Ewald real-space grid of configuration 1 with FP32 pair terms beyond 7 A (rg_grid_build_many_precision):
+    every one of the 46.6M points within 1e-3 K of the FP64 build on every channel, and 2,000 sampled points plus the
+    corners within the same bar of the oracle."""
+    print("case: config1 split-precision Ewald grid", flush=True)
+    fw, spec, dims, ff, f, grids, specs = config1_grids(gpu_ctx)
+    g64, g32 = grids[2], grids[0]
+    ra.build_grids(gpu_ctx, ff, f, [(g64, RG_GRID_EWALD_REAL, 0, 12.0)])
+    ra.build_grids(gpu_ctx, ff, f, [(g32, RG_GRID_EWALD_REAL, 0, 12.0)], split_radius=SPLIT_RADIUS)
+    n = 8 * dims[0] * dims[1] * dims[2]
+    chunk = 8 * dims[0] * dims[1] * 25
+    mx, s2 = np.zeros(8), np.zeros(8)
+    for off in range(0, n, chunk):
+        c = min(chunk, n - off)
+        m, r = channel_errors(g32.download(off, c), g64.download(off, c))
+        mx = np.maximum(mx, m)
+        s2 += r * r * (c // 8)
+    rms = np.sqrt(s2 / (n // 8))
+    print(f"split {SPLIT_RADIUS} A vs FP64, all points: max |d| K per channel {np.array2string(mx, precision=2)}, "
+          f"rms {np.array2string(rms, precision=2)}", flush=True)
+    assert np.all(mx <= 1e-3), mx
+    ijk = sample_points(dims, 2000, 6)
+    ref = oracle.grid_points(fw.cell, fw.rect, dims, 3, fw.pos, fw.type, fw.charge, oracle.pairs_from_spec(spec), 0,
+                             12.0, 12.0, spec.alpha, ijk, nthreads=NTHREADS)
+    got = gather(g32, dims, ijk)
+    m, r = channel_errors(got, ref)
+    print(f"split vs oracle, 2008 points: max |d| K {np.array2string(m, precision=2)}", flush=True)
+    assert np.all(np.abs(got - ref) <= ABS + REL * np.abs(ref))
+
+
+def test_split_precision_rejects_an_unfit_radius(gpu_ctx):
+    """A split radius whose alpha r range is too wide for the FP32 erfcx polynomial is refused, not approximated."""
+    fw, spec, dims, ff, f, grids, specs = config1_grids(gpu_ctx)
+    with pytest.raises(RGError, match="split radius"):
+        ra.build_grids(gpu_ctx, ff, f, [(grids[2], RG_GRID_EWALD_REAL, 0, 12.0)], split_radius=0.5)
 
 
 def test_fused_equals_single_and_is_deterministic(gpu_ctx):
