@@ -174,6 +174,24 @@ rg_status rg_grid_build_many_precision(rg_ctx* ctx, const rg_forcefield* ff, con
 rg_status rg_grid_download(rg_ctx* ctx, const rg_grid* grid, size_t offset, size_t count, double* host);
 rg_status rg_grid_upload(rg_ctx* ctx, rg_grid* grid, size_t offset, size_t count, const double* host);
 
+/* Grids on disk.  The reference rebuilds every grid at start and restart and archives only the grid's metadata
+ * (unit cell, origin, point and cell counts, order; src/raspakit/interpolation_energy_grid.cpp:4793-4836,
+ * src/raspakit/system.cpp:1382-1384).  rg_grid_save / rg_grid_load keep the built values in a binary file with that
+ * metadata and a 64-bit key of the inputs they were built from, so that a restart loads instead of rebuilding:
+ * rg_grid_load refuses (RG_ERR_INVALID) a file whose key, scheme, dimensions or cell differ from the grid's.
+ * No reference counterpart for the values in the file. */
+/* Key of a grid build: FNV-1a over the grid type, probe, supercell cutoff, precision and split radius, the framework's
+ * cell, atoms, types and charges, and the force field's pair table, cutoffs, alpha and overlap criterion. */
+rg_status rg_grid_key(const rg_forcefield* ff, const rg_framework* fw, rg_grid_type type, int probe_pseudo_atom,
+                      double cutoff_supercell, int precision, double split_radius, unsigned long long* key);
+rg_status rg_grid_save(rg_ctx* ctx, const rg_grid* grid, unsigned long long key, const char* path);
+rg_status rg_grid_load(rg_ctx* ctx, rg_grid* grid, unsigned long long key, const char* path);
+/* InterpolationEnergyGrid::writeOutput (src/raspakit/interpolation_energy_grid.cpp:4760-4790): the Gaussian-cube
+ * header the reference writes (origin 0; point counts negated; voxel vectors = lattice vectors / point counts; one
+ * dummy atom), then every stored value of the grid in memory order (all V values per point), in Kelvin, one per
+ * line, each as the shortest decimal that reads back to the same double. */
+rg_status rg_grid_write_cube(rg_ctx* ctx, const rg_grid* grid, const char* path);
+
 /* InterpolationEnergyGrid::interpolate (tricubic and triquintic branches,
  * src/raspakit/interpolation_energy_grid.cpp:3980-4180) for n positions (pos = n*3, host).  energy[n] receives the
  * interpolated energies; gradient (n*3) receives d/dr (interpolateGradient, :4189-4440) when non-null. */
@@ -321,7 +339,8 @@ typedef struct rg_mc_params
   double pressure_pa;       /* Pa; fugacity = mole fraction * pressure (fugacity coefficient 1) */
   int k_first;              /* numberOfFirstBeadPositions (reference default 10), <= 64 */
   int k_orient;             /* numberOfTrialDirections (reference default 10), <= 64 */
-  double p_translation;     /* move probabilities; the rest are CBMC swaps, half insertions, half deletions */
+  double p_translation;     /* move probabilities (with p_reinsertion below); the rest are CBMC swaps, half
+                             * insertions, half deletions */
   double p_rotation;
   double max_translation;   /* A */
   double max_rotation;      /* rad */
@@ -330,6 +349,8 @@ typedef struct rg_mc_params
   int max_molecules;        /* per system; each system's capacity_atoms must be >= max_molecules * max(n_atoms) */
   unsigned long long seed;  /* Philox4x32-10 key; trials are addressed by (system, step, slot) */
   long long production_cycles; /* cycles over which the 5 block averages are spread */
+  double p_reinsertion;     /* probability of a CBMC reinsertion (MC_Moves::reinsertionMove,
+                             * src/raspakit/mc_moves/component/reinsertion.cpp:34-214) */
 } rg_mc_params;
 
 typedef struct rg_mc_state_t
@@ -338,7 +359,7 @@ typedef struct rg_mc_state_t
   int n_per_component[2];
   double energy;            /* running adsorbate energy (internal units) */
   long long steps, cycles_sampled;
-  long long trials[4], accepted[4];  /* insertion, deletion, translation, rotation */
+  long long trials[5], accepted[5];  /* insertion, deletion, translation, rotation, reinsertion */
   double block_count[5], block_energy[5], block_n[5][2], block_energy_n[5][2], block_nn[5][2][2];
 } rg_mc_state_t;
 

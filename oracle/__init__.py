@@ -444,7 +444,7 @@ class MC:
                 cd[d0 + 36:d0 + 40] = d["torsion_K"]
                 cd[d0 + 40] = d.get("log_ideal_rosenbluth", 0.0)
         dpar = _f([params["temperature"], params["pressure_pa"], params["p_translation"], params["p_rotation"],
-                   params["max_translation"], params["max_rotation"]])
+                   params["max_translation"], params["max_rotation"], params.get("p_reinsertion", 0.0)])
         ipar = (C.c_longlong * 6)(params["k_first"], params["k_orient"], params["min_steps_per_cycle"],
                                   params["max_molecules"], params["production_cycles"], system)
         self._keep += [ci, cd, dpar, cell9, gcell9, pairs]
@@ -466,11 +466,11 @@ class MC:
         _mc_lib().orc_mc_run(self.h, int(cycles), int(sample))
 
     def state(self):
-        s = np.zeros(14)
+        s = np.zeros(16)
         b = np.zeros(50)
         _mc_lib().orc_mc_state(self.h, _d(s), _d(b))
         return dict(n_molecules=int(s[0]), n_per_component=[int(s[1]), int(s[2])], energy=s[3], steps=int(s[4]),
-                    cycles=int(s[5]), trials=s[6:10].astype(int), accepted=s[10:14].astype(int),
+                    cycles=int(s[5]), trials=s[6:11].astype(int), accepted=s[11:16].astype(int),
                     blocks=b.reshape(5, 10))
 
     def adsorbates(self, capacity=4096):

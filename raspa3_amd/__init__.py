@@ -307,6 +307,27 @@ def build_grids(ctx, ff, fw, specs, sync=True, split_radius=None):
     L.check(fn(ctx.h, ff.h, fw.h, n, L.i(types), L.i(probes), L.d(cuts), arr), ctx.h, "rg_grid_build_many")
 
 
+def grid_key(ff, fw, grid_type, probe, cutoff, split_radius=None):
+    """rg_grid_key: 64-bit key of the inputs a grid is built from (the check of a grid cache file)."""
+    k = C.c_ulonglong()
+    prec = L.RG_GRID_FP64 if split_radius is None else L.RG_GRID_SPLIT_FP32
+    L.check(lib().rg_grid_key(ff.h, fw.h, int(grid_type), int(probe), float(cutoff), prec,
+                              0.0 if split_radius is None else float(split_radius), C.byref(k)), None, "rg_grid_key")
+    return k.value
+
+
+def grid_save(ctx, grid, key, path):
+    L.check(lib().rg_grid_save(ctx.h, grid.h, C.c_ulonglong(key), str(path).encode()), ctx.h, "rg_grid_save")
+
+
+def grid_load(ctx, grid, key, path):
+    L.check(lib().rg_grid_load(ctx.h, grid.h, C.c_ulonglong(key), str(path).encode()), ctx.h, "rg_grid_load")
+
+
+def grid_write_cube(ctx, grid, path):
+    L.check(lib().rg_grid_write_cube(ctx.h, grid.h, str(path).encode()), ctx.h, "rg_grid_write_cube")
+
+
 def grid_check(ctx, ff, fw, lj_grid, probe, coulomb_grid=None, probe_charge=0.0, temperature=300.0,
                n_points=100000, frac=None, seed=1):
     """rg_grid_check: the reference's test of a built grid on random points (Boltzmann-weighted averages of the

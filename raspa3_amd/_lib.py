@@ -122,13 +122,14 @@ class McParams(C.Structure):
     _fields_ = [("temperature", C.c_double), ("pressure_pa", C.c_double), ("k_first", C.c_int),
                 ("k_orient", C.c_int), ("p_translation", C.c_double), ("p_rotation", C.c_double),
                 ("max_translation", C.c_double), ("max_rotation", C.c_double), ("min_steps_per_cycle", C.c_int),
-                ("max_molecules", C.c_int), ("seed", C.c_ulonglong), ("production_cycles", C.c_longlong)]
+                ("max_molecules", C.c_int), ("seed", C.c_ulonglong), ("production_cycles", C.c_longlong),
+                ("p_reinsertion", C.c_double)]
 
 
 class McState(C.Structure):
     _fields_ = [("n_molecules", C.c_int), ("n_per_component", C.c_int * 2), ("energy", C.c_double),
-                ("steps", C.c_longlong), ("cycles_sampled", C.c_longlong), ("trials", C.c_longlong * 4),
-                ("accepted", C.c_longlong * 4), ("block_count", C.c_double * 5), ("block_energy", C.c_double * 5),
+                ("steps", C.c_longlong), ("cycles_sampled", C.c_longlong), ("trials", C.c_longlong * 5),
+                ("accepted", C.c_longlong * 5), ("block_count", C.c_double * 5), ("block_energy", C.c_double * 5),
                 ("block_n", (C.c_double * 2) * 5), ("block_energy_n", (C.c_double * 2) * 5),
                 ("block_nn", ((C.c_double * 2) * 2) * 5)]
 
@@ -198,4 +199,12 @@ SIGNATURES.update({
     "rg_block_average": (C.c_int, [C.c_int, dp, dp, dp, dp]),
     "rg_enthalpy_of_adsorption": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, dp, dp, C.c_double, dp, dp]),
     "rg_widom_statistics": (C.c_int, [C.c_int, dp, dp, dp, C.c_double, C.POINTER(WidomStats)]),
+})
+
+
+SIGNATURES.update({
+    "rg_grid_key": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.POINTER(C.c_ulonglong)]),
+    "rg_grid_save": (C.c_int, [vp, vp, C.c_ulonglong, C.c_char_p]),
+    "rg_grid_load": (C.c_int, [vp, vp, C.c_ulonglong, C.c_char_p]),
+    "rg_grid_write_cube": (C.c_int, [vp, vp, C.c_char_p]),
 })

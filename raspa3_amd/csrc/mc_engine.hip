@@ -34,7 +34,22 @@ enum : uint32_t
   SLOT_DISP = 3,       // translation (2 slots) / rotation (2 slots)
   SLOT_FIRST = 16,     // first-bead trial j: slots 16 + 2j, 17 + 2j
   SLOT_ORIENT = 160,   // orientation trial j: slots 160 + 2j, 161 + 2j
-  SLOT_VOXEL = 304     // proposal trial j: slots 304 + 2j, 305 + 2j
+  SLOT_VOXEL = 304,    // proposal trial j: slots 304 + 2j, 305 + 2j
+  SLOT_REINS = 1u << 24  // offset of every slot of the retrace of a reinsertion (its trials are new draws)
+};
+
+// What a growth does.  retrace < 0 grows a new molecule; retrace = m retraces molecule m (its atoms are trial 0 of
+// every step).  'own' is the molecule left out of the background (-2: none).  'so' is added to every Philox slot of
+// the growth.  reins = 1 grows the new configuration of a reinsertion and sets logStoredR to the log of the summed
+// weights of the first-bead trials it did not select; reins = 2 retraces the old configuration of a reinsertion,
+// whose first-bead weight is that of its first bead alone plus exp(logStoredR) (Esselink et al., J. Chem. Phys. 101
+// (1994) 9033, eqs. 16 and 18; src/raspakit/cbmc/multiple_first_bead.cpp:78-136).
+struct GrowMode
+{
+  int retrace, own;
+  uint32_t so;
+  int reins;
+  double logStoredR;
 };
 
 __device__ __forceinline__ void uni(const McRun& R, int sys, unsigned long long step, uint32_t slot, double& a,
@@ -568,25 +583,25 @@ __device__ void waveAcceptEwald(const SysDev& S, int lane, const Team& T)
 // First-bead position of trial j: uniform in the cell or drawn from the proposal (alias table over voxels, uniform
 // inside the voxel).  Returns log(1 / (V q)) = -log(nvox p_v) (0 for the uniform distribution).
 __device__ double firstBeadTrial(const McRun& R, const SysDev& S, const McSys& Y, int sys, unsigned long long step,
-                                 int j, double p[3])
+                                 int j, double p[3], uint32_t so = 0)
 {
   double u0, u1, u2, u3;
   if (Y.prop.nvox > 0)
   {
     const Proposal& P = Y.prop;
-    uni(R, sys, step, SLOT_VOXEL + 2 * j, u0, u1);
-    uni(R, sys, step, SLOT_VOXEL + 2 * j + 1, u2, u3);
+    uni(R, sys, step, so + SLOT_VOXEL + 2 * j, u0, u1);
+    uni(R, sys, step, so + SLOT_VOXEL + 2 * j + 1, u2, u3);
     int col = min(int(u0 * P.nvox), P.nvox - 1);
     const int v = (u1 < P.aliasProb[col]) ? col : P.alias[col];
     const int vx = v % P.n[0], vy = (v / P.n[0]) % P.n[1], vz = v / (P.n[0] * P.n[1]);
     double w0, w1;
-    uni(R, sys, step, SLOT_FIRST + 2 * j, w0, w1);
+    uni(R, sys, step, so + SLOT_FIRST + 2 * j, w0, w1);
     const double f[3] = {(vx + u2) / P.n[0], (vy + u3) / P.n[1], (vz + w0) / P.n[2]};
     for (int c = 0; c < 3; ++c) p[c] = S.box.cell[c] * f[0] + S.box.cell[3 + c] * f[1] + S.box.cell[6 + c] * f[2];
     return -P.logNvoxP[v];
   }
-  uni(R, sys, step, SLOT_FIRST + 2 * j, u0, u1);
-  uni(R, sys, step, SLOT_FIRST + 2 * j + 1, u2, u3);
+  uni(R, sys, step, so + SLOT_FIRST + 2 * j, u0, u1);
+  uni(R, sys, step, so + SLOT_FIRST + 2 * j + 1, u2, u3);
   // SimulationBox::randomPosition: cell * (u, u, u) (src/raspakit/simulationbox.cpp:132-135)
   for (int c = 0; c < 3; ++c) p[c] = S.box.cell[c] * u0 + S.box.cell[3 + c] * u1 + S.box.cell[6 + c] * u2;
   return 0.0;
@@ -608,17 +623,20 @@ __device__ double proposalLogBias(const McSys& Y, const SysDev& S, const double 
 
 // First-bead step of a growth: kFirst trials (uniform or from the proposal; the existing first bead is trial 0 of a
 // retrace), Rosenbluth selection.  On success every lane holds the selected position, its energy and log weight,
-// and the second selection uniform of the move.
+// and the second selection uniform of the move.  The retrace of a reinsertion (G.reins == 2) evaluates the existing
+// first bead alone and adds the stored weight of the growth's unselected trials.
 __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                              unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& fx,
-                              double& fy, double& fz, double& Ufirst, double& logW1, double& uSel2)
+                              unsigned long long step, const GrowMode G, int lane, WaveScratch& W, const Team& T,
+                              double& fx, double& fy, double& fz, double& Ufirst, double& logW1, double& uSel2,
+                              double& logStoredR)
 {
   const int K1 = R.kFirst;
-  const int own = retrace >= 0 ? retrace : -2;
+  const int retrace = G.retrace;
+  const int Keval = G.reins == 2 ? 1 : K1;
   const int fb = c.first;
   double p[3] = {0.0, 0.0, 0.0};
   double logBias = 0.0;
-  if (lane < K1)
+  if (lane < Keval)
   {
     if (retrace >= 0 && lane == 0)
     {
@@ -629,7 +647,7 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
       logBias = proposalLogBias(Y, S, p);
     }
     else
-      logBias = firstBeadTrial(R, S, Y, sys, step, lane, p);
+      logBias = firstBeadTrial(R, S, Y, sys, step, lane, p, G.so);
     W.pos[3 * lane] = p[0];
     W.pos[3 * lane + 1] = p[1];
     W.pos[3 * lane + 2] = p[2];
@@ -638,13 +656,32 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
   double U1;
   bool ov1;
   const int fbIdx[1] = {fb};
-  trialSetEnergies(S, nads, c, fbIdx, 1, K1, W.pos, own, -2, lane, W, T, U1, ov1);
+  trialSetEnergies(S, nads, c, fbIdx, 1, Keval, W.pos, G.own, -2, lane, W, T, U1, ov1);
   double uSel1;
-  uni(R, sys, step, SLOT_SELECT, uSel1, uSel2);
+  uni(R, sys, step, G.so + SLOT_SELECT, uSel1, uSel2);
   int sel1;
-  waveRosenbluth(U1, ov1, logBias, K1, R.beta, uSel1, lane, sel1, logW1);
+  if (G.reins == 2)
+  {
+    // w(o) = exp(-beta u(o)) + r, divided by the number of trials (Esselink et al. eq. 18)
+    if (__shfl(ov1 ? 1 : 0, 0, 64)) return false;
+    const double a0 = __shfl(-R.beta * U1 + logBias, 0, 64);
+    const double hi = fmax(a0, G.logStoredR);
+    logW1 = hi + log(exp(a0 - hi) + exp(G.logStoredR - hi)) - log(double(K1));
+    sel1 = 0;
+  }
+  else
+    waveRosenbluth(U1, ov1, logBias, K1, R.beta, uSel1, lane, sel1, logW1);
   if (retrace >= 0) sel1 = 0;  // the retrace keeps the existing molecule
   if (sel1 < 0 || logW1 < R.logMinW) return false;
+  if (G.reins == 1)
+  {
+    // r = w(n) - exp(-beta U[selected]) (Esselink et al. eq. 16), as a log
+    const bool other = lane < K1 && !ov1 && lane != sel1;
+    const double lbf = other ? -R.beta * U1 + logBias : -INFINITY;
+    const double mx = rgd::waveMax(lbf);
+    const double sum = rgd::waveSum(other ? exp(lbf - mx) : 0.0);
+    logStoredR = (mx == -INFINITY) ? -INFINITY : mx + log(sum);
+  }
   fx = __shfl(p[0], sel1, 64);
   fy = __shfl(p[1], sel1, 64);
   fz = __shfl(p[2], sel1, 64);
@@ -658,14 +695,16 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
 // weight and the external energy of the grown molecule.  'retrace' >= 0 retraces that existing molecule instead
 // (trial 0 = its current first bead and orientation; its own atoms are excluded).
 __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                          unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& logW,
-                          double& Uext)
+                          unsigned long long step, const GrowMode G, int lane, WaveScratch& W, const Team& T,
+                          double& logW, double& Uext, double& logStoredR)
 {
   const int K2 = R.kOrient;
-  const int own = retrace >= 0 ? retrace : -2;
+  const int retrace = G.retrace;
+  const int own = G.own;
   const int fb = c.first;
   double fx, fy, fz, Ufirst, logW1, uSel2;
-  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, T, fx, fy, fz, Ufirst, logW1, uSel2)) return false;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, G, lane, W, T, fx, fy, fz, Ufirst, logW1, uSel2, logStoredR))
+    return false;
   if (c.na == 1)
   {
     if (lane == 0)
@@ -699,8 +738,8 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
     else
     {
       double u1, u2, u3, u4;
-      uni(R, sys, step, SLOT_ORIENT + 2 * lane, u1, u2);
-      uni(R, sys, step, SLOT_ORIENT + 2 * lane + 1, u3, u4);
+      uni(R, sys, step, G.so + SLOT_ORIENT + 2 * lane, u1, u2);
+      uni(R, sys, step, G.so + SLOT_ORIENT + 2 * lane + 1, u3, u4);
       randomRotation(u1, u2, u3, Rm);
       for (int s = 0; s < n; ++s)
       {
@@ -748,9 +787,9 @@ __device__ bool growRigid(const McRun& R, const SysDev& S, const McSys& Y, const
 constexpr uint32_t SLOT_CHAIN = 1024;
 
 __device__ __forceinline__ void uniChain(const McRun& R, int sys, unsigned long long step, int bead, int trial,
-                                         int kind, int attempt, double& a, double& b)
+                                         int kind, int attempt, double& a, double& b, uint32_t so = 0)
 {
-  uni(R, sys, step, SLOT_CHAIN + uint32_t(((bead * 64 + trial) * 4 + kind) * MC_MAXREJ + attempt), a, b);
+  uni(R, sys, step, so + SLOT_CHAIN + uint32_t(((bead * 64 + trial) * 4 + kind) * MC_MAXREJ + attempt), a, b);
 }
 
 __device__ __forceinline__ double chainTorsionEnergy(const Comp& c, double phi)
@@ -760,20 +799,21 @@ __device__ __forceinline__ double chainTorsionEnergy(const Comp& c, double phi)
 }
 
 // p(theta) ~ sin(theta) exp(-beta k (theta - theta0)^2 / 2): Gaussian proposal (Box-Muller), accepted with sin(theta)
-__device__ double sampleBend(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial)
+__device__ double sampleBend(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial,
+                             uint32_t so)
 {
   if (c.kbend <= 0.0)
   {
     double u, v;
-    uniChain(R, sys, step, bead, trial, 1, 0, u, v);
+    uniChain(R, sys, step, bead, trial, 1, 0, u, v, so);
     return acos(1.0 - 2.0 * u);
   }
   const double sd = 1.0 / sqrt(R.beta * c.kbend);
   for (int t = 0; t < MC_MAXREJ; ++t)
   {
     double g1, g2, a, b;
-    uniChain(R, sys, step, bead, trial, 1, t, g1, g2);
-    uniChain(R, sys, step, bead, trial, 2, t, a, b);
+    uniChain(R, sys, step, bead, trial, 1, t, g1, g2, so);
+    uniChain(R, sys, step, bead, trial, 2, t, a, b, so);
     const double th = c.theta0 + sd * sqrt(-2.0 * log(1.0 - g1)) * cos(2.0 * M_PI * g2);
     if (th > 0.0 && th < M_PI && a < sin(th)) return th;
   }
@@ -781,12 +821,13 @@ __device__ double sampleBend(const McRun& R, const Comp& c, int sys, unsigned lo
 }
 
 // p(phi) ~ exp(-beta U_tors(phi)) on [-pi, pi): uniform proposal accepted with exp(-beta (U - U_min))
-__device__ double sampleTorsion(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial)
+__device__ double sampleTorsion(const McRun& R, const Comp& c, int sys, unsigned long long step, int bead, int trial,
+                                uint32_t so)
 {
   for (int t = 0; t < MC_MAXREJ; ++t)
   {
     double u, a;
-    uniChain(R, sys, step, bead, trial, 3, t, u, a);
+    uniChain(R, sys, step, bead, trial, 3, t, u, a, so);
     const double phi = 2.0 * M_PI * u - M_PI;
     if (a < exp(-R.beta * (chainTorsionEnergy(c, phi) - c.torsMin))) return phi;
   }
@@ -810,12 +851,12 @@ __device__ __forceinline__ void cross3(const double a[3], const double b[3], dou
 
 // trial position of bead i given beads 0 .. i-1 (mol, 3 per bead): bond, sampled bend, azimuth or torsion
 __device__ void chainTrial(const McRun& R, const Comp& c, int sys, unsigned long long step, int i, int j,
-                           const double* mol, double out[3])
+                           const double* mol, double out[3], uint32_t so)
 {
   if (i == 1)
   {
     double u, v;
-    uniChain(R, sys, step, i, j, 0, 0, u, v);
+    uniChain(R, sys, step, i, j, 0, 0, u, v, so);
     const double z = 2.0 * u - 1.0, r = sqrt(fmax(0.0, 1.0 - z * z)), ps = 2.0 * M_PI * v;
     out[0] = mol[0] + c.bond * r * cos(ps);
     out[1] = mol[1] + c.bond * r * sin(ps);
@@ -842,16 +883,16 @@ __device__ void chainTrial(const McRun& R, const Comp& c, int sys, unsigned long
     unit3(n);
     cross3(n, e, m);
   }
-  const double th = sampleBend(R, c, sys, step, i, j);
+  const double th = sampleBend(R, c, sys, step, i, j, so);
   double ph;
   if (i == 2)
   {
     double u, v;
-    uniChain(R, sys, step, i, j, 0, 0, u, v);
+    uniChain(R, sys, step, i, j, 0, 0, u, v, so);
     ph = 2.0 * M_PI * u;
   }
   else
-    ph = sampleTorsion(R, c, sys, step, i, j);
+    ph = sampleTorsion(R, c, sys, step, i, j, so);
   const double ct = cos(th), st = sin(th), cp = cos(ph), sp = sin(ph);
   for (int d = 0; d < 3; ++d) out[d] = Cp[d] + c.bond * (-ct * e[d] + st * cp * m[d] + st * sp * n[d]);
 }
@@ -914,13 +955,15 @@ __device__ double chainIntra(const SysDev& S, const Comp& c, const double* mol, 
 // molecule, then kOrient trials for each of beads 1 .. na-1.  Writes the chain into W.mol; Uext is the external
 // plus intra-molecular (non-bonded and bonded) energy.
 __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys, int nads,
-                          unsigned long long step, int retrace, int lane, WaveScratch& W, const Team& T, double& logW,
-                          double& Uext)
+                          unsigned long long step, const GrowMode G, int lane, WaveScratch& W, const Team& T,
+                          double& logW, double& Uext, double& logStoredR)
 {
   const int K = R.kOrient;
-  const int own = retrace >= 0 ? retrace : -2;
+  const int retrace = G.retrace;
+  const int own = G.own;
   double fx, fy, fz, Ufirst, lw, uSel2;
-  if (!firstBeadStep(R, S, Y, c, sys, nads, step, retrace, lane, W, T, fx, fy, fz, Ufirst, lw, uSel2)) return false;
+  if (!firstBeadStep(R, S, Y, c, sys, nads, step, G, lane, W, T, fx, fy, fz, Ufirst, lw, uSel2, logStoredR))
+    return false;
   if (lane == 0)
   {
     W.mol[0] = fx;
@@ -943,7 +986,7 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
         p[2] = S.az[a0];
       }
       else
-        chainTrial(R, c, sys, step, i, lane, W.mol, p);
+        chainTrial(R, c, sys, step, i, lane, W.mol, p, G.so);
       W.pos[3 * lane] = p[0];
       W.pos[3 * lane + 1] = p[1];
       W.pos[3 * lane + 2] = p[2];
@@ -955,7 +998,7 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
     trialSetEnergies(S, nads, c, bi, 1, K, W.pos, own, -2, lane, W, T, Ui, ovi);
     if (lane < K) Ui += chainIntra(S, c, W.mol, i, p, ovi);
     double us, unused;
-    uni(R, sys, step, SLOT_CHAIN - 16 + i, us, unused);
+    uni(R, sys, step, G.so + SLOT_CHAIN - 16 + i, us, unused);
     int sel;
     waveRosenbluth(Ui, ovi, 0.0, K, R.beta, us, lane, sel, lw);
     if (retrace >= 0) sel = 0;
@@ -981,11 +1024,24 @@ __device__ bool growChain(const McRun& R, const SysDev& S, const McSys& Y, const
 // CHAIN = false compiles the chain growth out of kernels whose components are all rigid (fewer registers).
 template <bool CHAIN>
 __device__ __forceinline__ bool growMolecule(const McRun& R, const SysDev& S, const McSys& Y, const Comp& c, int sys,
-                                             int nads, unsigned long long step, int retrace, int lane, WaveScratch& W,
-                                             const Team& T, double& logW, double& Uext)
+                                             int nads, unsigned long long step, const GrowMode G, int lane,
+                                             WaveScratch& W, const Team& T, double& logW, double& Uext,
+                                             double& logStoredR)
 {
-  if (CHAIN && c.flexible) return growChain(R, S, Y, c, sys, nads, step, retrace, lane, W, T, logW, Uext);
-  return growRigid(R, S, Y, c, sys, nads, step, retrace, lane, W, T, logW, Uext);
+  if (CHAIN && c.flexible) return growChain(R, S, Y, c, sys, nads, step, G, lane, W, T, logW, Uext, logStoredR);
+  return growRigid(R, S, Y, c, sys, nads, step, G, lane, W, T, logW, Uext, logStoredR);
+}
+
+// A growth of a new molecule (retrace < 0) or the retrace of molecule 'retrace' for an insertion or deletion.
+__device__ __forceinline__ GrowMode swapMode(int retrace)
+{
+  GrowMode G;
+  G.retrace = retrace;
+  G.own = retrace >= 0 ? retrace : -2;
+  G.so = 0;
+  G.reins = 0;
+  G.logStoredR = -INFINITY;
+  return G;
 }
 
 
@@ -1032,7 +1088,9 @@ __global__ __launch_bounds__(256) void k_widom(const McRun* __restrict__ Rp, con
   const Comp& c = R.comp[comp];
   double logW = -INFINITY, U = 0.0, de = 0.0;
   const Team T{1, 0, &scratch[w]};
-  if (growMolecule<CHAIN>(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), -1, lane, scratch[w], T, logW, U))
+  double unusedR;
+  if (growMolecule<CHAIN>(R, S, Y, c, 0, S.nads, (unsigned long long)(first + i), swapMode(-1), lane, scratch[w], T,
+                          logW, U, unusedR))
     de = ewaldInsertRemove(R, S, S.netCharge, c, +1, lane, scratch[w], T, false);
   else
   {
@@ -1148,8 +1206,8 @@ struct Live
   double netCharge;
   long long step;
   long long cycle;
-  long long trials[4];
-  long long accepted[4];
+  long long trials[5];
+  long long accepted[5];
 };
 
 // Orientation-averaged Boltzmann factor of a rigid molecule with its first bead at each of n points (fractional
@@ -1344,11 +1402,18 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     }
     return;
   }
-  const double fugVol = R.beta * c.fugacity * S.box.volume;
-  // CBMC insertion (src/raspakit/mc_moves/component/insertion_cbmc.cpp:34-200) or deletion
-  // (src/raspakit/mc_moves/component/deletion_cbmc.cpp:67-174); one growth call serves both (retrace = m)
-  const bool insert = uMove < pRot + 0.5 * (1.0 - pRot);
-  L.trials[insert ? 0 : 1] += 1;
+  // CBMC insertion (src/raspakit/mc_moves/component/insertion_cbmc.cpp:34-200), deletion
+  // (src/raspakit/mc_moves/component/deletion_cbmc.cpp:67-174) or reinsertion (MC_Moves::reinsertionMove,
+  // src/raspakit/mc_moves/component/reinsertion.cpp:34-214) of a molecule of the component.  One growth call site
+  // serves all three: an insertion grows, a deletion retraces molecule m, a reinsertion regrows m with its own atoms
+  // left out of the background (pass 0) and then retraces its old configuration with the growth's unselected
+  // first-bead trials (pass 1).  Kernels with flexible chains (CHAIN) are built without the reinsertion: with it,
+  // hipcc (ROCm 7.2) fails on k_gcmc<true> with "Illegal instruction detected: Operand has incorrect register class"
+  // (a V_CMP on src_shared_base), so the host refuses p_reinsertion > 0 for them.
+  const double pRe = pRot + R.pReinsertion;
+  const bool reins = !CHAIN && uMove < pRe;
+  const bool insert = !reins && uMove < pRe + 0.5 * (1.0 - pRe);
+  L.trials[reins ? 4 : (insert ? 0 : 1)] += 1;
   const int nc = L.ncomp[ci];
   int m = -1;
   if (insert)
@@ -1361,8 +1426,72 @@ __device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sy
     m = kthMoleculeOfComponent(Y, L.nmol, ci, min(int(uMol * nc), nc - 1), lane);
     if (m < 0) return;
   }
-  double logW, Uext;
-  if (!growMolecule<CHAIN>(R, S, Y, c, sys, L.nads, step, insert ? -1 : m, lane, W, T, logW, Uext)) return;
+  double logW = 0.0, Uext = 0.0, logWn = 0.0, Un = 0.0, logStoredR = -INFINITY;
+  double nx = 0.0, ny = 0.0, nz = 0.0;  // new configuration of a reinsertion (lane a holds atom a)
+  for (int pass = 0; pass < (reins ? 2 : 1); ++pass)
+  {
+    GrowMode G = swapMode(insert ? -1 : m);
+    if (reins)
+    {
+      G.retrace = pass == 0 ? -1 : m;
+      G.so = pass == 0 ? 0u : uint32_t(SLOT_REINS);
+      G.reins = pass + 1;
+      G.logStoredR = logStoredR;
+    }
+    double stored = -INFINITY;
+    if (!growMolecule<CHAIN>(R, S, Y, c, sys, L.nads, step, G, lane, W, T, logW, Uext, stored)) return;
+    if (reins && pass == 0)
+    {
+      logStoredR = stored;
+      logWn = logW;
+      Un = Uext;
+      if (lane < c.na)
+      {
+        nx = W.mol[3 * lane];
+        ny = W.mol[3 * lane + 1];
+        nz = W.mol[3 * lane + 2];
+      }
+      waveSyncLds();
+    }
+  }
+  if (reins)
+  {
+    double de = 0.0;
+    const bool ew = S.useCharge && c.charged;
+    if (ew)
+    {
+      // Fourier difference of the old atoms (-q) and the new ones (+q), as for a translation
+      const int src = lane >= c.na ? lane - c.na : lane;
+      const double sx = __shfl(nx, src, 64), sy = __shfl(ny, src, 64), sz = __shfl(nz, src, 64);
+      if (lane < 2 * c.na)
+      {
+        const bool isNew = lane >= c.na;
+        const int a0 = m * R.slots + src;
+        W.at[4 * lane] = isNew ? sx : S.ax[a0];
+        W.at[4 * lane + 1] = isNew ? sy : S.ay[a0];
+        W.at[4 * lane + 2] = isNew ? sz : S.az[a0];
+        W.at[4 * lane + 3] = isNew ? c.q[src] : -c.q[src];
+      }
+      waveSyncLds();
+      de = waveEwaldDelta(S, 2 * c.na, W.at, W.tab, lane, true, W, T);
+    }
+    // Pacc = exp(-beta dE_Fourier) W(new) / W(old) (reinsertion.cpp:189-190); logW, Uext: the old configuration
+    if (log(uAcc) < -R.beta * de + logWn - logW)
+    {
+      if (ew) waveAcceptEwald(S, lane, T);
+      if (T.wi == 0 && lane < c.na)
+      {
+        const int a0 = m * R.slots + lane;
+        S.ax[a0] = nx;
+        S.ay[a0] = ny;
+        S.az[a0] = nz;
+      }
+      L.U += Un - Uext + de;
+      L.accepted[4] += 1;
+    }
+    return;
+  }
+  const double fugVol = R.beta * c.fugacity * S.box.volume;
   const double de = ewaldInsertRemove(R, S, L.netCharge, c, insert ? +1 : -1, lane, W, T, true);
   if (insert)
   {
@@ -1462,7 +1591,7 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
   L.netCharge = S.netCharge;
   L.step = Y.step;
   L.cycle = Y.cycle;
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 5; ++q)
   {
     L.trials[q] = Y.stat.trials[q];
     L.accepted[q] = Y.stat.accepted[q];
@@ -1512,7 +1641,7 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
     Sg.netCharge = L.netCharge;
     Yg.step = L.step;
     Yg.cycle = L.cycle;
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < 5; ++q)
     {
       Yg.stat.trials[q] = L.trials[q];
       Yg.stat.accepted[q] = L.accepted[q];

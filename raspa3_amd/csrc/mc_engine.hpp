@@ -58,8 +58,8 @@ struct McBlock
 
 struct McStat
 {
-  long long trials[4];    // insertion, deletion, translation, rotation
-  long long accepted[4];
+  long long trials[5];    // insertion, deletion, translation, rotation, reinsertion
+  long long accepted[5];
 };
 
 // Per-system mutable Monte Carlo state (device).
@@ -88,7 +88,7 @@ struct McRun
   int maxMolecules;
   int minStepsPerCycle;
   int slots;              // atom slots per molecule in the adsorbate arrays: the largest n_atoms of the components
-  double pTranslation, pRotation;  // swap probability = 1 - both, insertion/deletion 50/50
+  double pTranslation, pRotation, pReinsertion;  // swap probability = 1 - all three, insertion/deletion 50/50
   double maxTranslation, maxRotation;
   long long productionCycles;
   Comp comp[MC_MAXCOMP];

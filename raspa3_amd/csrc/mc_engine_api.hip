@@ -67,6 +67,10 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
   R.minStepsPerCycle = std::max(1, p->min_steps_per_cycle);
   R.pTranslation = p->p_translation;
   R.pRotation = p->p_rotation;
+  R.pReinsertion = p->p_reinsertion;
+  if (!(R.pTranslation >= 0.0 && R.pRotation >= 0.0 && R.pReinsertion >= 0.0 &&
+        R.pTranslation + R.pRotation + R.pReinsertion <= 1.0))
+    return fail(ctx, RG_ERR_INVALID, "move probabilities must be >= 0 and sum to at most 1");
   R.maxTranslation = p->max_translation;
   R.maxRotation = p->max_rotation;
   R.productionCycles = std::max(1LL, p->production_cycles);
@@ -111,6 +115,10 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
     }
     C.eIntra = ff->use_charge ? self + excl : 0.0;
     C.flexible = d.flexible ? 1 : 0;
+    if (C.flexible && R.pReinsertion > 0.0)
+      return fail(ctx, RG_ERR_UNSUPPORTED,
+                  "component %d: the reinsertion move is built for rigid molecules only (runs with a flexible chain "
+                  "use the kernel without it)", c);
     if (C.flexible)
     {
       if (d.first_bead != 0 || C.charged || !(d.bond_length > 0.0) || d.bend_k_K < 0.0)
@@ -412,7 +420,7 @@ rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* o
   out->energy = Y.U;
   out->steps = Y.step;
   out->cycles_sampled = Y.cycle;
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 5; ++q)
   {
     out->trials[q] = Y.stat.trials[q];
     out->accepted[q] = Y.stat.accepted[q];

@@ -343,11 +343,11 @@ def methane_component(spec):
 
 def mc_params(temperature=300.0, pressure_pa=1e5, seed=42, k_first=10, k_orient=10, p_translation=0.25,
               p_rotation=0.25, max_translation=0.5, max_rotation=0.5, min_steps_per_cycle=20, max_molecules=256,
-              production_cycles=1000):
+              production_cycles=1000, p_reinsertion=0.0):
     return dict(temperature=temperature, pressure_pa=pressure_pa, k_first=k_first, k_orient=k_orient,
                 p_translation=p_translation, p_rotation=p_rotation, max_translation=max_translation,
                 max_rotation=max_rotation, min_steps_per_cycle=min_steps_per_cycle, max_molecules=max_molecules,
-                seed=seed, production_cycles=production_cycles)
+                seed=seed, production_cycles=production_cycles, p_reinsertion=p_reinsertion)
 
 
 def grid_dims(cell, spacing):

@@ -373,3 +373,46 @@ def test_grid_check_matches_oracle(gpu_ctx, case, scheme):
     # the default test: 1e5 Philox points, as the reference runs it after every build
     d = ra.grid_check(gpu_ctx, ff, f, lj, probe, coul, qp, 300.0, n_points=100000, seed=5)
     assert d["n_points"] == 100000 and np.isfinite(d["lennard_jones"]["energy_relative_error"])
+
+
+def test_grid_cache_round_trip_and_cube_export(gpu_ctx, tmp_path):
+    """rg_grid_save / rg_grid_load give back the built grid bit for bit and refuse a file built from other inputs;
+    rg_grid_write_cube writes the reference's cube header and every stored value in memory order, in K
+    (InterpolationEnergyGrid::writeOutput, interpolation_energy_grid.cpp:4760-4790)."""
+    cell, rect = w.cell_from_lengths_angles(12.0, 13.0, 12.5, 85.0, 95.0, 100.0)
+    rng = np.random.default_rng(8)
+    pos = rng.uniform(0, 1, (24, 3)) @ cell.T
+    typ = np.array([0, 1, 1] * 8, np.int32)
+    q = np.array([2.05, -1.025, -1.025] * 8)
+    fw = w.Framework("cache", cell, rect, pos, typ, q)
+    spec = w.forcefield_config1()
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    dims = [11, 12, 9]
+    g = ra.Grid(gpu_ctx, cell, rect, dims)
+    ra.build_grids(gpu_ctx, ff, f, [(g, RG_GRID_EWALD_REAL, 0, 12.0)])
+    key = ra.grid_key(ff, f, RG_GRID_EWALD_REAL, 0, 12.0)
+    assert key != ra.grid_key(ff, f, RG_GRID_EWALD_REAL, 0, 12.0, split_radius=7.0)
+    assert key != ra.grid_key(ff, f, RG_GRID_LENNARD_JONES, spec.index("O_co2"), 12.0)
+    path = tmp_path / "ewald.grid"
+    ra.grid_save(gpu_ctx, g, key, path)
+    h = ra.Grid(gpu_ctx, cell, rect, dims)
+    ra.grid_load(gpu_ctx, h, key, path)
+    assert np.array_equal(h.download(), g.download())
+    with pytest.raises(RGError, match="different grid"):
+        ra.grid_load(gpu_ctx, h, key + 1, path)
+    with pytest.raises(RGError, match="different grid"):
+        ra.grid_load(gpu_ctx, ra.Grid(gpu_ctx, cell, rect, [11, 12, 10]), key, path)
+    cube = tmp_path / "grid.cube"
+    ra.grid_write_cube(gpu_ctx, g, cube)
+    lines = cube.read_text().splitlines()
+    assert lines[0] == "Cube interpolation file" and lines[1] == "Written by RASPA-3"
+    assert [float(x) for x in lines[2].split()] == [1.0, 0.0, 0.0, 0.0]
+    for a in range(3):
+        row = lines[3 + a].split()
+        assert int(row[0]) == -dims[a]
+        assert np.allclose([float(x) for x in row[1:]], cell[:, a] / dims[a], rtol=0, atol=0)
+    assert lines[6] == "1 0.0 0.0 0.0 0.0"
+    vals = np.array([float(x) for x in lines[7:]])
+    assert vals.size == 8 * np.prod(dims)
+    assert np.allclose(vals, g.download() * oracle.units()["energy_to_kelvin"], rtol=1e-15, atol=0)

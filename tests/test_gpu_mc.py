@@ -197,6 +197,46 @@ def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx, scheme):
     mc.close()
 
 
+def test_gcmc_reinsertion_replays_oracle_trajectory(gpu_ctx):
+    """CBMC reinsertion (MC_Moves::reinsertionMove, src/raspakit/mc_moves/component/reinsertion.cpp:34-214) on the
+    device against the oracle's replay: CO2/N2 80:20 in charged ITQ-29 with Ewald kmax 6, 30 % reinsertions,
+    a tabulated first-bead proposal on one of the two systems (the stored weight carries the proposal's bias)."""
+    print("case: gcmc reinsertion replay", flush=True)
+    spec = w.forcefield_zeolite([("N_n2", 36.0, 3.31, -0.482), ("N_com", 0.0, 1.0, 0.964)])
+    types = [spec.index(n) for n in ("C_co2", "O_co2", "N_n2", "N_com")]
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    comps = [w.co2_reference(spec, 0.8), w.n2_component(spec, 0.2)]
+    params = w.mc_params(pressure_pa=5e5, seed=91, max_molecules=128, production_cycles=10, p_translation=0.2,
+                         p_rotation=0.2, p_reinsertion=0.3)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                         capacity=8 * 128) for _ in range(2)]
+    mc = ra.Mc(gpu_ctx, systems, comps, params)
+    n3 = [8, 8, 8]
+    prob = np.random.default_rng(5).uniform(0.2, 1.0, 512)
+    mc.set_proposal(1, n3, prob)
+    mc.run(4, sample=False, graph=True)
+    mc.run(10, sample=True, graph=True)
+    for sidx in range(2):
+        g = compare_with_oracle(mc, sidx, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, unit, (4, 10),
+                                proposal=(n3, prob) if sidx == 1 else None)
+        assert g["trials"][4] > 100 and g["accepted"][4] > 10
+    tot = mc.total_energy()
+    for sidx in range(2):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
+
+
+def test_reinsertion_of_flexible_chains_is_refused(gpu_ctx):
+    """Kernels with flexible chains are built without the reinsertion move: p_reinsertion > 0 with a chain component
+    is refused on the host instead of being ignored."""
+    spec = w.forcefield_hexane_zeolite()
+    ff = ra.ForceField(gpu_ctx, spec)
+    s = ra.System(gpu_ctx, ff, np.diag([30.0] * 3), True, kmax=(0, 0, 0), capacity=64)
+    with pytest.raises(ra.RGError, match="reinsertion"):
+        ra.Mc(gpu_ctx, [s], [w.hexane_component(spec)], w.mc_params(p_reinsertion=0.1))
+
+
 def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
     """CO2/N2 80:20 in charged ITQ-29 with a tabulated first-bead proposal (random positive voxel probabilities)."""
     print("case: gcmc mixture + proposal replay", flush=True)

@@ -198,3 +198,83 @@ def test_gcmc_hexane_follows_henry_law():
     mc.run(8000, sample=True)
     b = mc.state()["blocks"]
     assert abs((b[:, 2] / b[:, 0]).mean() / 0.25 - 1.0) < 0.1
+
+
+def test_reinsertion_moves_sample_boltzmann_configurations():
+    """Pins the CBMC reinsertion of the oracle (Esselink's stored weight of the growth's unselected first-bead trials,
+    src/raspakit/cbmc/multiple_first_bead.cpp:78-136, and the acceptance W(new) / W(old), reinsertion.cpp:189-190).
+    The same target as the rotation test: an uncharged CO2 in ITQ-29 at <N> ~ 0.3, <U>/<N> against the Boltzmann
+    average over position and orientation from quadrature, with reinsertions making 70 % of the moves (translation
+    and rotation 5 % each).  Leaving the stored weight out of the retrace, or keeping the selected trial in it, biases
+    the reinsertion towards low or high energies and moves this average."""
+    spec = w.forcefield_zeolite()
+    spec.use_charge = False
+    fw, unit = w.framework_itq29(spec, 3)
+    dims = w.grid_dims(unit, 0.25)[0]
+    pairs = oracle.pairs_from_spec(spec)
+    g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
+                              spec.alpha, nthreads=8) for t in (spec.index("C_co2"), spec.index("O_co2"))}
+    comp = w.co2_reference(spec)
+    comp["charge"] = [0.0, 0.0, 0.0]
+    e2k = oracle.units()["energy_to_kelvin"]
+    beta = 1.0 / (oracle.units()["kB"] * 300.0)
+    q = oracle.MC(spec, fw, [comp], w.mc_params(), kmax=(0, 0, 0), grids=(g, None, dims, unit))
+    frac = np.random.default_rng(2).random((60000, 3))
+    lw, mu = q.molecule_weights(0, frac, 16, 6, mean_energy=True)
+    W = np.exp(lw)
+    u_ref = (W * mu).sum() / W.sum() * e2k
+    err_ref = np.std([(W[i::10] * mu[i::10]).sum() / W[i::10].sum() for i in range(10)], ddof=1) / np.sqrt(10) * e2k
+    vol = abs(np.linalg.det(fw.cell))
+    pconv = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+    P = 0.3 / (beta * vol * W.mean()) * pconv
+    mc = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=P, p_translation=0.05, p_rotation=0.05,
+                                                 p_reinsertion=0.7, production_cycles=20000, seed=5),
+                   kmax=(0, 0, 0), grids=(g, None, dims, unit))
+    mc.run(200)
+    mc.run(20000, sample=True)
+    s = mc.state()
+    b = s["blocks"]
+    per = b[:, 1] / b[:, 2] * e2k
+    u_mc, err_mc = per.mean(), per.std(ddof=1) / np.sqrt(len(per))
+    assert s["accepted"][4] > 5000 and s["trials"][4] > 2 * s["accepted"][4]
+    assert abs(u_mc - u_ref) < 4.0 * np.hypot(err_mc, err_ref) + 10.0, (u_mc, err_mc, u_ref, err_ref)
+
+
+def test_reinsertion_of_chains_samples_rosenbluth_average():
+    """Chain reinsertion (the growth and retrace of a flexible n-hexane, with Esselink's stored first-bead weight):
+    at <N> ~ 0.25 in ITQ-29, <U>/<N> from the block sums equals the Rosenbluth-weighted average sum_i W_i U_i /
+    sum_i W_i over independent growths (Widom insertions), which is the Boltzmann average of the grown molecule's
+    energy; reinsertions make 70 % of the moves."""
+    spec = w.forcefield_hexane_zeolite()
+    spec.use_charge = False
+    e = oracle.MC(spec, empty_box(), [w.hexane_component(spec)], w.mc_params(seed=4), kmax=(0, 0, 0))
+    wig = np.exp(e.widom(0, 0, 40000)[0]).mean()
+    comp = w.hexane_component(spec, np.log(wig))
+    fw, unit = w.framework_itq29(spec, 3)
+    dims = w.grid_dims(unit, 0.25)[0]
+    pairs = oracle.pairs_from_spec(spec)
+    g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
+                              spec.alpha, nthreads=8) for t in (spec.index("CH3"), spec.index("CH2"))}
+    grids = (g, None, dims, unit)
+    lw, u, _ = oracle.MC(spec, fw, [comp], w.mc_params(seed=8), kmax=(0, 0, 0), grids=grids).widom(0, 0, 40000)
+    W = np.exp(lw - np.max(lw[np.isfinite(lw)]))
+    ok = np.isfinite(lw)
+    e2k = oracle.units()["energy_to_kelvin"]
+    u_ref = (W[ok] * u[ok]).sum() / W[ok].sum() * e2k
+    parts = [(W[i::10][ok[i::10]] * u[i::10][ok[i::10]]).sum() / W[i::10][ok[i::10]].sum() for i in range(10)]
+    err_ref = np.std(parts, ddof=1) / np.sqrt(10) * e2k
+    Wm = np.exp(lw).mean() / wig
+    beta = 1.0 / (oracle.units()["kB"] * 300.0)
+    pconv = 1.6605402e-27 / (1e-10 * 1e-12 * 1e-12)
+    P = 0.25 / (beta * abs(np.linalg.det(fw.cell)) * Wm) * pconv
+    mc = oracle.MC(spec, fw, [comp], w.mc_params(pressure_pa=P, p_translation=0.05, p_rotation=0.05,
+                                                 p_reinsertion=0.7, production_cycles=8000, seed=23),
+                   kmax=(0, 0, 0), grids=grids)
+    mc.run(100)
+    mc.run(8000, sample=True)
+    s = mc.state()
+    b = s["blocks"]
+    per = b[:, 1] / b[:, 2] * e2k
+    u_mc, err_mc = per.mean(), per.std(ddof=1) / np.sqrt(len(per))
+    assert s["accepted"][4] > 2000
+    assert abs(u_mc - u_ref) < 4.0 * np.hypot(err_mc, err_ref) + 20.0, (u_mc, err_mc, u_ref, err_ref)
