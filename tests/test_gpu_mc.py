@@ -232,7 +232,7 @@ def test_reinsertion_of_flexible_chains_is_refused(gpu_ctx):
     is refused on the host instead of being ignored."""
     spec = w.forcefield_hexane_zeolite()
     ff = ra.ForceField(gpu_ctx, spec)
-    s = ra.System(gpu_ctx, ff, np.diag([30.0] * 3), True, kmax=(0, 0, 0), capacity=64)
+    s = ra.System(gpu_ctx, ff, np.diag([30.0] * 3), True, kmax=(0, 0, 0), capacity=8 * 256)
     with pytest.raises(ra.RGError, match="reinsertion"):
         ra.Mc(gpu_ctx, [s], [w.hexane_component(spec)], w.mc_params(p_reinsertion=0.1))
 
