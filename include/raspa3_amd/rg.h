@@ -384,9 +384,13 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
 /* Releases the engine.  The systems hold the adsorbates, Ewald state and net charge of the last rg_mc_run, so they
  * may be released before or after the engine (but not used by the engine once released). */
 rg_status rg_mc_destroy(rg_mc* mc);
-/* Learned first-bead proposal for one system: probability[v] > 0 over n3[0]*n3[1]*n3[2] voxels of the cell (x fastest);
- * NULL restores the uniform proposal. */
+/* Learned first-bead proposal for one system, the same for every component: probability[v] > 0 over
+ * n3[0]*n3[1]*n3[2] voxels of the cell (x fastest); NULL restores the uniform proposal.  No reference counterpart
+ * (the reference's first beads are uniform, src/raspakit/cbmc/multiple_first_bead.cpp:26-27). */
 rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability);
+/* The same for one component of one system (each adsorbate its own proposal); no reference counterpart. */
+rg_status rg_mc_set_component_proposal(rg_ctx* ctx, rg_mc* mc, int system, int component, const int* n3,
+                                       const double* probability);
 /* Runs 'cycles' MC cycles; sample != 0 accumulates block averages; use_graph != 0 replays one captured cycle.
  * On return the systems' adsorbate arrays, structure factors, adsorbate count and net charge are the final state. */
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph);

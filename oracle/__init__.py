@@ -372,7 +372,7 @@ def _mc_lib():
                                     C.c_double, C.c_int, C.c_double, C.c_int, dp, ip, dp, C.POINTER(dp), dp, dp, ip,
                                     ip, C.c_longlong, C.c_int, ip, dp, dp, C.POINTER(C.c_longlong), C.c_ulonglong]
         L.orc_mc_destroy.argtypes = [vp]
-        L.orc_mc_set_proposal.argtypes = [vp, ip, dp]
+        L.orc_mc_set_proposal.argtypes = [vp, C.c_int, ip, dp]
         L.orc_mc_run.argtypes = [vp, C.c_int, C.c_int]
         L.orc_mc_state.argtypes = [vp, dp, dp]
         L.orc_mc_adsorbates.argtypes = [vp, dp, ip, ip]
@@ -457,10 +457,10 @@ class MC:
                                  _i(gdims) if gdims is not None else None, _i(_n(kmax)), int(int_cut2), len(comps),
                                  _i(ci), _d(cd), _d(dpar), ipar, int(params["seed"]))
 
-    def set_proposal(self, n3, prob):
+    def set_proposal(self, n3, prob, component=-1):
         p = _f(prob).reshape(-1)
         self._keep.append(p)
-        _mc_lib().orc_mc_set_proposal(self.h, _i(_n(n3)), _d(p))
+        _mc_lib().orc_mc_set_proposal(self.h, int(component), _i(_n(n3)), _d(p))
 
     def run(self, cycles, sample=False):
         _mc_lib().orc_mc_run(self.h, int(cycles), int(sample))

@@ -492,10 +492,15 @@ class Mc(_Handle):
                 "rg_mc_create")
         self._register(ctx, h)
 
-    def set_proposal(self, system, n3, prob):
+    def set_proposal(self, system, n3, prob, component=None):
+        """First-bead proposal of a system: for every component, or for one (component index)."""
         n = L.i32(n3)
         p = None if prob is None else L.f64(prob).reshape(-1)
-        L.check(lib().rg_mc_set_proposal(self.ctx.h, self.h, system, L.i(n), L.d(p)), self.ctx.h, "set_proposal")
+        if component is None:
+            L.check(lib().rg_mc_set_proposal(self.ctx.h, self.h, system, L.i(n), L.d(p)), self.ctx.h, "set_proposal")
+        else:
+            L.check(lib().rg_mc_set_component_proposal(self.ctx.h, self.h, system, int(component), L.i(n), L.d(p)),
+                    self.ctx.h, "set_component_proposal")
 
     def set_launch_shape(self, shape):
         """-1: autotuned; 0, 1, 2: one wavefront per system, 1, 2 or 4 systems per workgroup; 3: four per system."""

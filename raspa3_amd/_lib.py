@@ -159,6 +159,7 @@ SIGNATURES.update({
     "rg_mc_destroy": (C.c_int, [vp]),
     "rg_mc_set_launch_shape": (C.c_int, [vp, vp, C.c_int]),
     "rg_mc_set_proposal": (C.c_int, [vp, vp, C.c_int, ip, dp]),
+    "rg_mc_set_component_proposal": (C.c_int, [vp, vp, C.c_int, C.c_int, ip, dp]),
     "rg_mc_run": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),
     "rg_mc_state": (C.c_int, [vp, vp, C.c_int, C.POINTER(McState)]),
     "rg_mc_adsorbates": (C.c_int, [vp, vp, C.c_int, C.c_int, dp, ip, ip, ip]),

@@ -582,13 +582,13 @@ __device__ void waveAcceptEwald(const SysDev& S, int lane, const Team& T)
 
 // First-bead position of trial j: uniform in the cell or drawn from the proposal (alias table over voxels, uniform
 // inside the voxel).  Returns log(1 / (V q)) = -log(nvox p_v) (0 for the uniform distribution).
-__device__ double firstBeadTrial(const McRun& R, const SysDev& S, const McSys& Y, int sys, unsigned long long step,
-                                 int j, double p[3], uint32_t so = 0)
+__device__ double firstBeadTrial(const McRun& R, const SysDev& S, const McSys& Y, int ci, int sys,
+                                 unsigned long long step, int j, double p[3], uint32_t so = 0)
 {
   double u0, u1, u2, u3;
-  if (Y.prop.nvox > 0)
+  if (Y.prop[ci].nvox > 0)
   {
-    const Proposal& P = Y.prop;
+    const Proposal& P = Y.prop[ci];
     uni(R, sys, step, so + SLOT_VOXEL + 2 * j, u0, u1);
     uni(R, sys, step, so + SLOT_VOXEL + 2 * j + 1, u2, u3);
     int col = min(int(u0 * P.nvox), P.nvox - 1);
@@ -607,10 +607,10 @@ __device__ double firstBeadTrial(const McRun& R, const SysDev& S, const McSys& Y
   return 0.0;
 }
 
-__device__ double proposalLogBias(const McSys& Y, const SysDev& S, const double p[3])
+__device__ double proposalLogBias(const McSys& Y, int ci, const SysDev& S, const double p[3])
 {
-  if (Y.prop.nvox <= 0) return 0.0;
-  const Proposal& P = Y.prop;
+  if (Y.prop[ci].nvox <= 0) return 0.0;
+  const Proposal& P = Y.prop[ci];
   int idx[3];
   for (int a = 0; a < 3; ++a)
   {
@@ -644,10 +644,10 @@ __device__ bool firstBeadStep(const McRun& R, const SysDev& S, const McSys& Y, c
       p[0] = S.ax[a0];
       p[1] = S.ay[a0];
       p[2] = S.az[a0];
-      logBias = proposalLogBias(Y, S, p);
+      logBias = proposalLogBias(Y, c.index, S, p);
     }
     else
-      logBias = firstBeadTrial(R, S, Y, sys, step, lane, p, G.so);
+      logBias = firstBeadTrial(R, S, Y, c.index, sys, step, lane, p, G.so);
     W.pos[3 * lane] = p[0];
     W.pos[3 * lane + 1] = p[1];
     W.pos[3 * lane + 2] = p[2];
@@ -1136,7 +1136,7 @@ __global__ __launch_bounds__(256) void k_widom_single(const McRun* __restrict__ 
   {
     step = static_cast<unsigned long long>(first + i);
     double p[3];
-    logBias = firstBeadTrial(R, S, Y, 0, step, k, p);
+    logBias = firstBeadTrial(R, S, Y, comp, 0, step, k, p);
     for (int a = 0; a < S.nads; ++a)  // the same atom for every lane: uniform loads
     {
       const int at = S.at[a];
@@ -1255,7 +1255,7 @@ __global__ __launch_bounds__(256) void k_molecule_weights(const SysDev* __restri
 }
 
 // Index of the kth molecule (in slot order) of component ci among the first nmol, or -1: 64 slots per ballot.
-__device__ int kthMoleculeOfComponent(const McSys& Y, int nmol, int ci, int kth, int lane)
+__device__ __forceinline__ int kthMoleculeOfComponent(const McSys& Y, int nmol, int ci, int kth, int lane)
 {
   int seen = 0;
   for (int base = 0; base < nmol; base += 64)
@@ -1274,8 +1274,10 @@ __device__ int kthMoleculeOfComponent(const McSys& Y, int nmol, int ci, int kth,
   return -1;
 }
 
+// Inlined into k_gcmc: as an outlined function (the compiler's choice once the step grew) it reached the kernel's LDS
+// descriptors and scratch through generic pointers, one flat access per field.
 template <bool CHAIN>
-__device__ void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W,
+__device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W,
                          const Team& T)
 {
   reloadBarrier();

@@ -20,7 +20,7 @@ struct Comp
   int na;
   int first;
   int charged;
-  int pad;
+  int index;          // position of the component in the run (selects its first-bead proposal)
   double ref[MC_MAXATOMS][3];
   int type[MC_MAXATOMS];
   double q[MC_MAXATOMS];
@@ -72,7 +72,7 @@ struct McSys
   long long step;         // move counter = Philox stream position
   long long cycle;        // production cycles sampled
   int* comp;              // component of each molecule slot
-  Proposal prop;
+  Proposal prop[MC_MAXCOMP];  // first-bead proposal per component (nvox = 0: uniform)
   McStat stat;
   McBlock block[MC_BLOCKS];
 };

@@ -114,6 +114,7 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
       }
     }
     C.eIntra = ff->use_charge ? self + excl : 0.0;
+    C.index = c;
     C.flexible = d.flexible ? 1 : 0;
     if (C.flexible && R.pReinsertion > 0.0)
       return fail(ctx, RG_ERR_UNSUPPORTED,
@@ -233,28 +234,31 @@ rg_status rg_mc_destroy(rg_mc* mc)
   return st;
 }
 
-rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability)
+}  // extern "C"
+
+namespace
 {
-  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || (probability && !n3))
-    return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: invalid arguments");
-  RG_ENTER(ctx);
+
+// Installs the tabulated first-bead proposal of one component of one system (probability NULL: uniform again).  The
+// tables it replaces are released once the device descriptor points at the new ones.
+rg_status setProposal(rg_ctx* ctx, rg_mc* mc, int system, int comp, const int* n3, const double* probability,
+                      const char* who)
+{
   McSys& Y = mc->hY[system];
-  if (mc->proposalTables.size() < mc->systems.size()) mc->proposalTables.resize(mc->systems.size(), {{nullptr, nullptr, nullptr}});
-  const std::array<void*, 3> replaced = mc->proposalTables[system];
+  const size_t slot = size_t(system) * MC_MAXCOMP + comp;
+  if (mc->proposalTables.size() < mc->systems.size() * MC_MAXCOMP)
+    mc->proposalTables.resize(mc->systems.size() * MC_MAXCOMP, {{nullptr, nullptr, nullptr}});
+  const std::array<void*, 3> replaced = mc->proposalTables[slot];
   std::array<void*, 3> fresh = {{nullptr, nullptr, nullptr}};
-  if (!probability)
-  {
-    Y.prop = Proposal{};
-  }
-  else
+  Proposal P{};
+  if (probability)
   {
     const int nvox = n3[0] * n3[1] * n3[2];
-    if (nvox <= 0 || n3[0] <= 0 || n3[1] <= 0 || n3[2] <= 0)
-      return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: bad voxel grid");
+    if (nvox <= 0 || n3[0] <= 0 || n3[1] <= 0 || n3[2] <= 0) return fail(ctx, RG_ERR_INVALID, "%s: bad voxel grid", who);
     double sum = 0.0;
     for (int v = 0; v < nvox; ++v)
     {
-      if (!(probability[v] > 0.0)) return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: probabilities must be > 0");
+      if (!(probability[v] > 0.0)) return fail(ctx, RG_ERR_INVALID, "%s: probabilities must be > 0", who);
       sum += probability[v];
     }
     // Walker / Vose alias table of the normalised probabilities
@@ -283,10 +287,6 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
     }
     for (int v : large) prob[v] = 1.0, alias[v] = v;
     for (int v : small) prob[v] = 1.0, alias[v] = v;
-    auto release = [&fresh]() {
-      for (void* p : fresh)
-        if (p) (void)hipFree(p);
-    };
     const size_t bytes[3] = {nvox * sizeof(double), nvox * sizeof(int), nvox * sizeof(double)};
     for (int t = 0; t < 3; ++t)
     {
@@ -294,30 +294,60 @@ rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, 
       if (e != hipSuccess)
       {
         fresh[t] = nullptr;
-        release();
-        return hipfail(ctx, e, "rg_mc_set_proposal: hipMalloc");
+        for (void* q : fresh)
+          if (q) (void)hipFree(q);
+        return hipfail(ctx, e, who);
       }
     }
-    void *dp = fresh[0], *da = fresh[1], *dl = fresh[2];
-    mc->proposalTables[system] = fresh;
-    RG_HIP(ctx, hipMemcpyAsync(dp, prob.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    RG_HIP(ctx, hipMemcpyAsync(da, alias.data(), nvox * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    RG_HIP(ctx, hipMemcpyAsync(dl, logp.data(), nvox * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    Y.prop.nvox = nvox;
-    for (int a = 0; a < 3; ++a) Y.prop.n[a] = n3[a];
-    Y.prop.aliasProb = static_cast<const double*>(dp);
-    Y.prop.alias = static_cast<const int*>(da);
-    Y.prop.logNvoxP = static_cast<const double*>(dl);
+    mc->proposalTables[slot] = fresh;
+    RG_HIP(ctx, hipMemcpyAsync(fresh[0], prob.data(), bytes[0], hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(ctx, hipMemcpyAsync(fresh[1], alias.data(), bytes[1], hipMemcpyHostToDevice, ctx->stream));
+    RG_HIP(ctx, hipMemcpyAsync(fresh[2], logp.data(), bytes[2], hipMemcpyHostToDevice, ctx->stream));
+    P.nvox = nvox;
+    for (int a = 0; a < 3; ++a) P.n[a] = n3[a];
+    P.aliasProb = static_cast<const double*>(fresh[0]);
+    P.alias = static_cast<const int*>(fresh[1]);
+    P.logNvoxP = static_cast<const double*>(fresh[2]);
   }
+  else
+    mc->proposalTables[slot] = fresh;
+  Y.prop[comp] = P;
   // only the proposal descriptor changes: copy it into the device state
-  if (!probability) mc->proposalTables[system] = fresh;
-  RG_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char*>(mc->dY + system) + offsetof(McSys, prop), &Y.prop,
-                             sizeof(Proposal), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char*>(mc->dY + system) + offsetof(McSys, prop) + comp * sizeof(Proposal),
+                             &Y.prop[comp], sizeof(Proposal), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   // no kernel reads the replaced tables any more: the device descriptor points at the new ones
-  for (void* p : replaced)
-    if (p) RG_HIP(ctx, hipFree(p));
+  for (void* q : replaced)
+    if (q) RG_HIP(ctx, hipFree(q));
   return RG_OK;
+}
+
+}  // namespace
+
+extern "C"
+{
+
+rg_status rg_mc_set_proposal(rg_ctx* ctx, rg_mc* mc, int system, const int* n3, const double* probability)
+{
+  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || (probability && !n3))
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_set_proposal: invalid arguments");
+  RG_ENTER(ctx);
+  for (int c = 0; c < mc->run.ncomp; ++c)
+  {
+    const rg_status st = setProposal(ctx, mc, system, c, n3, probability, "rg_mc_set_proposal");
+    if (st != RG_OK) return st;
+  }
+  return RG_OK;
+}
+
+rg_status rg_mc_set_component_proposal(rg_ctx* ctx, rg_mc* mc, int system, int component, const int* n3,
+                                       const double* probability)
+{
+  if (!ctx || !mc || system < 0 || system >= int(mc->systems.size()) || component < 0 ||
+      component >= mc->run.ncomp || (probability && !n3))
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_set_component_proposal: invalid arguments");
+  RG_ENTER(ctx);
+  return setProposal(ctx, mc, system, component, n3, probability, "rg_mc_set_component_proposal");
 }
 
 rg_status rg_mc_set_launch_shape(rg_ctx* ctx, rg_mc* mc, int shape)

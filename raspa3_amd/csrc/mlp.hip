@@ -8,9 +8,9 @@
 // Weights are kept as FP32 masters (updated by Adam) and rounded to BF16 for the products; activations are stored
 // in BF16 for the next product and pre-activations in FP32 for the backward pass.
 //
-// GEMM tile: 64 x 64 outputs per workgroup, four wavefronts of 32 x 32 (2 x 2 MFMA 16 x 16 x 32 blocks), K steps
-// of 32 staged through LDS with the K index contiguous, so every MFMA operand is one 16-byte LDS read; the global
-// loads of the next K step are issued before the products of the current one.
+// GEMM tile: 64 x 64 outputs per workgroup, four wavefronts of 32 x 32 (2 x 2 MFMA 16 x 16 x 32 blocks); the whole K
+// extent of a stage (up to 256) is staged through LDS with the K index contiguous, so every MFMA operand is one
+// 16-byte LDS read, and all of a stage's global loads are in flight together.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -43,15 +43,18 @@ enum Epilogue
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // C (M x N) = op(A) (M x K) * op(B) (K x N).  TA: A is stored K x M; TB: B is stored N x K.  All row-major.
-// Workgroup (x, y, z) computes the 64 x 64 tile (y, x) over k in [z Kc, min(K, (z + 1) Kc)).
+// Workgroup (x, y, z) computes the 64 x 64 tile (y, x) over k in [z Kc, min(K, (z + 1) Kc)).  K is taken in stages
+// of up to KS = 256: every global load of a stage is issued before any of them is written to LDS, so a stage costs
+// one memory latency (the network's K extents, 192 and 256, and the weight-gradient chunks are one stage each).
+constexpr int KS = 256;
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const __bf16* __restrict__ A, int lda,
                                               const __bf16* __restrict__ B, int ldb, const float* __restrict__ bias,
                                               const float* __restrict__ Zprev, float* __restrict__ outF,
                                               __bf16* __restrict__ outH)
 {
-  __shared__ __attribute__((aligned(16))) __bf16 As[64][40];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][40];
+  __shared__ __attribute__((aligned(16))) __bf16 As[64][KS + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][KS + 8];
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
@@ -59,67 +62,59 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int kEnd = min(K, int(blockIdx.z + 1) * Kc);
-  // this thread's 8-element slices of the A and B tiles of a K step: read from global memory one step ahead (the
-  // next step's loads are in flight while the current step's products run), then written to LDS
-  auto loadA = [&](int k0) -> bf16x8
+  for (int k0 = blockIdx.z * Kc; k0 < kEnd; k0 += KS)
   {
-    if (!TA)
-      return *reinterpret_cast<const bf16x8*>(A + size_t(m0 + (t >> 2)) * lda + k0 + (t & 3) * 8);
-    return *reinterpret_cast<const bf16x8*>(A + size_t(k0 + (t >> 3)) * lda + m0 + (t & 7) * 8);
-  };
-  auto loadB = [&](int k0) -> bf16x8
-  {
-    if (!TB) return *reinterpret_cast<const bf16x8*>(B + size_t(k0 + (t >> 3)) * ldb + n0 + (t & 7) * 8);
-    return *reinterpret_cast<const bf16x8*>(B + size_t(n0 + (t >> 2)) * ldb + k0 + (t & 3) * 8);
-  };
-  auto storeA = [&](const bf16x8& v)
-  {
-    if (!TA)
-      *reinterpret_cast<bf16x8*>(&As[t >> 2][(t & 3) * 8]) = v;
-    else
-    {
-      const int kk = t >> 3, mm = (t & 7) * 8;
+    const int kk = min(KS, kEnd - k0);  // a multiple of 32
+    const int nv = 8 * kk;               // 8-element vectors per operand tile (64 rows x kk)
+    const int rowv = kk / 8;             // vectors along k per row of a k-contiguous operand
+    bf16x8 va[KS / 32], vb[KS / 32];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) As[mm + i][kk] = v[i];
-    }
-  };
-  auto storeB = [&](const bf16x8& v)
-  {
-    if (!TB)
+    for (int q = 0; q < KS / 32; ++q)
     {
-      const int kk = t >> 3, nn = (t & 7) * 8;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) Bs[nn + i][kk] = v[i];
-    }
-    else
-      *reinterpret_cast<bf16x8*>(&Bs[t >> 2][(t & 3) * 8]) = v;
-  };
-  const int kBegin = blockIdx.z * Kc;
-  bf16x8 va, vb;
-  if (kBegin < kEnd)
-  {
-    va = loadA(kBegin);
-    vb = loadB(kBegin);
-  }
-  for (int k0 = kBegin; k0 < kEnd; k0 += 32)
-  {
-    storeA(va);
-    storeB(vb);
-    __syncthreads();
-    if (k0 + 32 < kEnd)
-    {
-      va = loadA(k0 + 32);
-      vb = loadB(k0 + 32);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-    {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
+      const int v = t + 256 * q;
+      if (v < nv)
       {
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + (lane & 15)][8 * (lane >> 4)]);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+        va[q] = !TA ? *reinterpret_cast<const bf16x8*>(A + size_t(m0 + v / rowv) * lda + k0 + (v % rowv) * 8)
+                    : *reinterpret_cast<const bf16x8*>(A + size_t(k0 + (v >> 3)) * lda + m0 + (v & 7) * 8);
+        vb[q] = !TB ? *reinterpret_cast<const bf16x8*>(B + size_t(k0 + (v >> 3)) * ldb + n0 + (v & 7) * 8)
+                    : *reinterpret_cast<const bf16x8*>(B + size_t(n0 + v / rowv) * ldb + k0 + (v % rowv) * 8);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KS / 32; ++q)
+    {
+      const int v = t + 256 * q;
+      if (v < nv)
+      {
+        if (!TA)
+          *reinterpret_cast<bf16x8*>(&As[v / rowv][(v % rowv) * 8]) = va[q];
+        else
+        {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) As[(v & 7) * 8 + i][v >> 3] = va[q][i];
+        }
+        if (!TB)
+        {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) Bs[(v & 7) * 8 + i][v >> 3] = vb[q][i];
+        }
+        else
+          *reinterpret_cast<bf16x8*>(&Bs[v / rowv][(v % rowv) * 8]) = vb[q];
+      }
+    }
+    __syncthreads();
+    for (int ks = 0; ks < kk; ks += 32)
+    {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+      {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + (lane & 15)][ks + 8 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+        {
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + (lane & 15)][ks + 8 * (lane >> 4)]);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+        }
       }
     }
     __syncthreads();

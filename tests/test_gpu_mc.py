@@ -58,6 +58,46 @@ def test_widom_methane_itq29_grid(gpu_ctx):
     assert W > 1.0  # methane adsorbs in the LTA cages
 
 
+def test_widom_config0_replays_oracle_with_chemical_potential(gpu_ctx):
+    """The CPU-runnable workload of the reference's Widom path: CH4 in the 288-site orthorhombic silicate
+    (20.0 x 19.9 x 13.4 A), CH4 grid at 0.2 A (100 x 99 x 67), 1e4 Widom insertions of 10 first-bead trials, seed 42,
+    300 K.  Every insertion's Rosenbluth weight and energy replay the oracle; the excess chemical potential and its
+    95 % block interval (5 blocks, rg_widom_statistics) equal the oracle's statistics of the oracle's weights.  The
+    synthetic framework is non-porous for CH4 (no point of the cell lies more than 2.6 A from a framework atom), so
+    <W> (~1e-17) is a rare-event average that 1e5 trials do not converge: the move-by-move replay, not a comparison
+    with a quadrature of exp(-beta U), is what pins it here (the porous ITQ-29 Widom tests compare with Henry's law
+    in the oracle's tests)."""
+    print("case: widom CH4 config 0 (1e4 insertions, seed 42)", flush=True)
+    from oracle import statistics as so
+    fw, spec = w.framework_config0(), w.forcefield_config0()
+    ff = ra.ForceField(gpu_ctx, spec)
+    f = ra.Framework(gpu_ctx, fw)
+    dims = ra.grid_dims(fw.cell, 0.2)
+    ch4 = spec.index("CH4")
+    g = ra.Grid(gpu_ctx, fw.cell, fw.rect, dims)
+    ra.build_grids(gpu_ctx, ff, f, [(g, RG_GRID_LENNARD_JONES, ch4, 12.0)])
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids={ch4: g}, kmax=(0, 0, 0), capacity=64)
+    comp = w.methane_component(spec)
+    params = w.mc_params(seed=42)
+    n = 10000
+    lw, u, _ = ra.widom(gpu_ctx, s, comp, params, 0, n)
+    host = g.download().reshape(dims[2], dims[1], dims[0], 8)
+    om = oracle.MC(spec, fw, [comp], params, kmax=(0, 0, 0), grids=({ch4: host}, None, dims))
+    olw, ou, _ = om.widom(0, 0, n)
+    assert close(lw, olw) and close(u, ou)
+    wts = np.exp(lw).reshape(5, -1)
+    ow = np.exp(olw).reshape(5, -1)
+    blocks = np.full(5, float(wts.shape[1]))
+    vol = abs(np.linalg.det(fw.cell))
+    st = ra.widom_statistics(blocks, wts.sum(1), np.zeros(5) + 1.0 / vol * blocks, 300.0)
+    ost = so.widom_statistics(blocks, ow.sum(1), np.zeros(5) + 1.0 / vol * blocks, 300.0)
+    e2k = oracle.units()["energy_to_kelvin"]
+    print(f"config 0 Widom: <W> = {st['rosenbluth_weight']:.6e} +- {st['rosenbluth_weight_ci']:.2e}, mu_ex = "
+          f"{st['mu_excess'] * e2k:.3f} +- {st['mu_excess_ci'] * e2k:.3f} K", flush=True)
+    assert st["mu_excess"] == pytest.approx(ost["mu_excess"], rel=1e-9)
+    assert st["mu_excess_ci"] == pytest.approx(ost["mu_excess_ci"], rel=1e-6)
+
+
 def test_widom_methane_into_loaded_system(gpu_ctx):
     """Widom insertions of CH4 into ITQ-29 holding CH4 molecules from a replayed GCMC run: the single-atom Widom
     kernel (several insertions per wavefront) with inter-molecular LJ, against the oracle holding the same
@@ -156,7 +196,9 @@ def itq29_setup(ctx, spec, types, spacing=0.3, scheme=8):
 def compare_with_oracle(mc, sysidx, spec, fw, comps, params, kmax, host, hcoul, dims, gcell, cycles, proposal=None):
     om = oracle.MC(spec, fw, comps, params, kmax=kmax, system=sysidx, grids=(host, hcoul, dims, gcell))
     if proposal is not None:
-        om.set_proposal(*proposal)
+        # (n3, prob) for every component, or a list of (component, n3, prob)
+        for item in (proposal if isinstance(proposal, list) else [(-1,) + tuple(proposal)]):
+            om.set_proposal(item[1], item[2], component=item[0])
     om.run(cycles[0])
     om.run(cycles[1], sample=True)
     o = om.state()
@@ -200,7 +242,8 @@ def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx, scheme):
 def test_gcmc_reinsertion_replays_oracle_trajectory(gpu_ctx):
     """CBMC reinsertion (MC_Moves::reinsertionMove, src/raspakit/mc_moves/component/reinsertion.cpp:34-214) on the
     device against the oracle's replay: CO2/N2 80:20 in charged ITQ-29 with Ewald kmax 6, 30 % reinsertions,
-    a tabulated first-bead proposal on one of the two systems (the stored weight carries the proposal's bias)."""
+    a tabulated first-bead proposal of its own for each component of one of the two systems (the stored weight
+    carries the proposal's bias)."""
     print("case: gcmc reinsertion replay", flush=True)
     spec = w.forcefield_zeolite([("N_n2", 36.0, 3.31, -0.482), ("N_com", 0.0, 1.0, 0.964)])
     types = [spec.index(n) for n in ("C_co2", "O_co2", "N_n2", "N_com")]
@@ -212,13 +255,15 @@ def test_gcmc_reinsertion_replays_oracle_trajectory(gpu_ctx):
                          capacity=8 * 128) for _ in range(2)]
     mc = ra.Mc(gpu_ctx, systems, comps, params)
     n3 = [8, 8, 8]
-    prob = np.random.default_rng(5).uniform(0.2, 1.0, 512)
-    mc.set_proposal(1, n3, prob)
+    rng = np.random.default_rng(5)
+    props = [(0, n3, rng.uniform(0.2, 1.0, 512)), (1, [6, 7, 8], rng.uniform(0.2, 1.0, 336))]
+    for comp_index, dims3, prob in props:
+        mc.set_proposal(1, dims3, prob, component=comp_index)
     mc.run(4, sample=False, graph=True)
     mc.run(10, sample=True, graph=True)
     for sidx in range(2):
         g = compare_with_oracle(mc, sidx, spec, fw, comps, params, (6, 6, 6), host, hcoul, dims, unit, (4, 10),
-                                proposal=(n3, prob) if sidx == 1 else None)
+                                proposal=props if sidx == 1 else None)
         assert g["trials"][4] > 100 and g["accepted"][4] > 10
     tot = mc.total_energy()
     for sidx in range(2):
