@@ -431,7 +431,8 @@ rg_status rg_block_average(int n_blocks, const double* sums, const double* weigh
 
 /* EnthalpyOfAdsorptionTerms (src/raspakit/properties/enthalpy.ixx:101-200) block-averaged as a composite property
  * (property_block_average.ixx:108-126): per block, from the weighted sums of U, N_i, U N_i and N_i N_j (row-major
- * [block][i] and [block][i][j], as in rg_mc_state_t), Delta H_i = sum_j (<U N_j> - <U><N_j>) (M^-1)_ji - k_B T with
+ * [block][i] and [block][i][j] with n_components columns; rg_mc_state_t's arrays have this layout for two
+ * components, for one component take column 0), Delta H_i = sum_j (<U N_j> - <U><N_j>) (M^-1)_ji - k_B T with
  * M_ij = <N_i N_j> - <N_i><N_j>; mean[i] = the mean over the blocks, ci[i] its interval.  Internal energy units
  * (the isosteric heat is -Delta H); temperature in K.  A singular block matrix gives NaN. */
 rg_status rg_enthalpy_of_adsorption(int n_blocks, int n_components, const double* weight, const double* sum_u,
