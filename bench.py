@@ -36,9 +36,10 @@ Second half of the metric, in the same JSON line:
   cycle.  No option runs it: its launch-shape tuning pass took a GPU memory access fault whose cause has not been
   found.
 * gcmc_mixture: CO2/N2 80:20 GCMC at 300 K and 1e5 Pa in 256 systems of uneven size (charged ITQ-29 supercells of
-  2592-15552 sites, Ewald kmax 8), one launch per cycle, plain CBMC against the learned first-bead proposal (trained
-  on the unit cell, tiled over each supercell): accepted insertions/s, loading per cell of both components and the
-  isosteric heat, and whether they agree within 2 sigma.
+  2592-15552 sites, Ewald kmax 8) in balanced launches (every system at least one cycle per launch, the smaller ones
+  more while the most loaded runs one), plain CBMC against the learned first-bead proposal (one network per
+  adsorbate, trained on the unit cell, tiled over each supercell): accepted insertions/s, loading per cell of both
+  components and the isosteric heat, and whether they agree within 2 sigma.
 * grids_triquintic: the three grids of the headline workload in the triquintic scheme (27 values per point, one
   launch per grid), per-kernel HIP-event times, FP64 FLOP rates, and the grid test with Hessians.
 """
@@ -671,7 +672,8 @@ def bench_gcmc_uneven(ctx, args):
 
 def bench_gcmc_mixture(ctx, args):
     """Combined workload: CO2/N2 80:20 GCMC in many systems of uneven size (charged ITQ-29 supercells of 2592-15552
-    sites), one launch per cycle, plain CBMC against the learned first-bead proposal.  Each adsorbate has a proposal
+    sites) in balanced launches (smaller systems run extra cycles while the largest runs one), plain CBMC against the
+    learned first-bead proposal.  Each adsorbate has a proposal
     network of its own, trained on that molecule's orientation-averaged Boltzmann factors in the 11.87 A unit cell,
     whose table is tiled over each supercell (any positive density keeps the acceptance rule exact)."""
     import raspa3_amd as ra
@@ -726,6 +728,9 @@ def bench_gcmc_mixture(ctx, args):
         params = w.mc_params(temperature=T, pressure_pa=P, seed=seed, max_molecules=mmax, k_first=k_first,
                              production_cycles=args.mixture_production)
         mc = ra.Mc(ctx, systems, comps, params)
+        if not args.no_mixture_balance:
+            # lightly loaded (smaller) supercells run extra whole cycles while the most loaded one runs its first
+            mc.set_balance(True, args.mixture_balance_overhead)
         if learned:
             for k, sh in enumerate(shape_of):
                 for ci in range(len(comps)):
@@ -747,6 +752,7 @@ def bench_gcmc_mixture(ctx, args):
         tot = mc.total_energy()
         drift = max(abs(tot[k] - st1[k]["energy"]) for k in range(nsys))
         n_final = np.array([st["n_per_component"] for st in st1])
+        ncyc = np.array([st["cycles"] for st in st1])
         mc.close()
         for s in systems:
             s.close()
@@ -755,6 +761,7 @@ def bench_gcmc_mixture(ctx, args):
                  insertion_acceptance=r["accepted_insertions"] / max(1, r["insertion_trials"]),
                  co2_per_cell_by_block=[round(float(x), 4) for x in by_block],
                  molecules_final=[int(n_final.sum(1).min()), int(n_final.sum(1).max())],
+                 production_cycles_per_system=[int(ncyc.min()), round(float(ncyc.mean()), 1), int(ncyc.max())],
                  max_energy_drift=float(drift))
         log(f"gcmc mixture {label}: {r['accepted_insertions_per_s']:.4e} accepted insertions/s, acc "
             f"{r['insertion_acceptance']:.4f}, CO2/N2 per cell {r['loading_per_cell'][0]:.4f} +- "
@@ -774,10 +781,14 @@ def bench_gcmc_mixture(ctx, args):
             qst_within_2sigma=within_2sigma(r, plain, "qst_kJmol", "qst_sigma"))
         extra["learned_over_plain_same_k_first"] = (learned["accepted_insertions_per_s"] /
                                                    r["accepted_insertions_per_s"])
-    return dict(**extra,workload=f"CO2/N2 80:20 at {T:g} K and {P:g} Pa, {nsys} systems in charged ITQ-29 supercells "
-                         f"{shapes} (2592-15552 sites), Ewald kmax 8, {args.mixture_equilibration} + "
-                         f"{args.mixture_production} cycles, grids at 0.2 A on the unit cell; plain CBMC with 10 "
-                         f"first-bead trials, the learned proposal with {args.mixture_kfirst}",
+    balance = ("one cycle per system and launch" if args.no_mixture_balance else
+               f"balanced launches (each system at least one cycle per launch, more while its cost, steps x "
+               f"(molecules + {args.mixture_balance_overhead:g}), stays within the largest single-cycle cost)")
+    return dict(**extra, workload=f"CO2/N2 80:20 at {T:g} K and {P:g} Pa, {nsys} systems in charged ITQ-29 "
+                                  f"supercells {shapes} (2592-15552 sites), Ewald kmax 8, "
+                                  f"{args.mixture_equilibration} + {args.mixture_production} launches, {balance}, "
+                                  f"grids at 0.2 A on the unit cell; plain CBMC with 10 first-bead trials, the "
+                                  f"learned proposal with {args.mixture_kfirst}",
                 plain=plain, learned=learned,
                 learned_over_plain=learned["accepted_insertions_per_s"] / plain["accepted_insertions_per_s"],
                 loading_within_2sigma=within_2sigma(learned, plain, "loading_per_cell", "loading_per_cell_sigma"),
@@ -820,8 +831,45 @@ def bench_mlp(ctx, g1, args):
                               frac=tflops / BF16_PEAK_TFLOPS))
 
 
+def parallel_timed(nthreads, make, step, target_s):
+    """Runs make(i) then step(obj) repeatedly on nthreads host threads (the oracle's ctypes calls release the GIL),
+    from a common start until each thread has run target_s seconds; returns (objects, per-thread counts, wall s)."""
+    import threading
+
+    objs = [None] * nthreads
+    counts = [0] * nthreads
+    errors = []
+    ready = threading.Barrier(nthreads + 1)
+    done = threading.Barrier(nthreads + 1)
+
+    def work(i):
+        try:
+            objs[i] = make(i)
+        except Exception as e:  # reported after the barrier
+            errors.append(e)
+        ready.wait()
+        t0 = time.perf_counter()
+        while not errors and time.perf_counter() - t0 < target_s:
+            counts[i] += step(objs[i])
+        done.wait()
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(nthreads)]
+    for t in threads:
+        t.start()
+    ready.wait()
+    t0 = time.perf_counter()
+    done.wait()
+    wall = time.perf_counter() - t0
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return objs, counts, wall
+
+
 def cpu_baseline_gcmc(target_s=10.0):
-    """The oracle's plain GCMC replay of one system (one host thread), same workload as the GPU run."""
+    """The oracle's plain GCMC of the same workload as the GPU run, one independent system per host thread (each
+    with its own seed), 100 untimed cycles first; accepted insertions of all threads over the wall time."""
     import oracle
     from raspa3_amd import workloads as w
 
@@ -829,32 +877,36 @@ def cpu_baseline_gcmc(target_s=10.0):
     fw, unit = w.framework_itq29(spec, 3)
     dims = w.grid_dims(unit, 0.2)[0]
     pairs = oracle.pairs_from_spec(spec)
+    cores = host_cores()
     g = {t: oracle.grid_build(unit, True, dims, 0, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, t, 12.0, 12.0,
-                              spec.alpha, nthreads=host_cores())
+                              spec.alpha, nthreads=cores)
          for t in (spec.index("C_co2"), spec.index("O_co2"))}
     coul = oracle.grid_build(unit, True, dims, 3, fw.pos[:72], fw.type[:72], fw.charge[:72], pairs, 0, 12.0, 12.0,
-                             spec.alpha, nthreads=host_cores())
+                             spec.alpha, nthreads=cores)
     _, kmax = oracle.ewald_auto(1e-6, 12.0, np.diag(fw.cell))
-    mc = oracle.MC(spec, fw, [w.co2_reference(spec)], w.mc_params(seed=3, max_molecules=256),
-                   kmax=tuple(int(k) for k in kmax), grids=(g, coul, dims, unit))
-    mc.run(100)  # loads the system part of the way
-    a0 = mc.state()["accepted"][0]
-    cyc, dt = 0, 0.0
-    while dt < target_s:
-        t = time.perf_counter()
-        mc.run(20)
-        dt += time.perf_counter() - t
-        cyc += 20
-    acc = mc.state()["accepted"][0] - a0
-    return dict(value=acc / dt, unit="accepted insertions/s", cores=1, kind="port",
-                sample=f"oracle plain GCMC, 1 system, cycles 100-{100 + cyc} ({acc} accepted insertions in "
-                       f"{dt:.1f} s, N = {mc.state()['n_molecules']}); systems are independent, so C host cores "
-                       f"give at most C times this")
+
+    def make(i):
+        mc = oracle.MC(spec, fw, [w.co2_reference(spec)], w.mc_params(seed=3 + i, max_molecules=256),
+                       kmax=tuple(int(k) for k in kmax), grids=(g, coul, dims, unit))
+        mc.run(100)  # loads the system part of the way
+        mc.base = mc.state()["accepted"][0]
+        return mc
+
+    def step(mc):
+        mc.run(5)
+        return 5
+
+    mcs, cycles, wall = parallel_timed(cores, make, step, target_s)
+    acc = sum(int(mc.state()["accepted"][0] - mc.base) for mc in mcs)
+    nmol = [mc.state()["n_molecules"] for mc in mcs]
+    return dict(value=acc / wall, unit="accepted insertions/s", cores=cores, kind="port",
+                sample=f"oracle plain GCMC, {cores} independent systems on {cores} threads, {sum(cycles)} cycles after "
+                       f"100 untimed ones ({acc} accepted insertions in {wall:.1f} s, N = {min(nmol)}-{max(nmol)})")
 
 
 def cpu_baseline_widom(target_s=10.0):
-    """The oracle's Widom insertions of CH4 in configuration 0 (one host thread), the trial-energy workload of the GPU
-    run: grid at 0.2 A, 10 first-bead trials per insertion."""
+    """The oracle's Widom insertions of CH4 in configuration 0, the trial-energy workload of the GPU run (grid at
+    0.2 A, 10 first-bead trials per insertion), on every host thread (disjoint insertion ranges of one seed)."""
     import oracle
     from raspa3_amd import workloads as w
 
@@ -862,21 +914,28 @@ def cpu_baseline_widom(target_s=10.0):
     pairs = oracle.pairs_from_spec(spec)
     t = spec.index("CH4")
     dims = w.grid_dims(fw.cell, 0.2)[0]
+    cores = host_cores()
     g = oracle.grid_build(fw.cell, fw.rect, dims, 0, fw.pos, fw.type, fw.charge, pairs, t, 12.0, 12.0, spec.alpha,
-                          nthreads=host_cores())
+                          nthreads=cores)
     params = w.mc_params(seed=7)
-    mc = oracle.MC(spec, fw, [w.methane_component(spec)], params, kmax=(0, 0, 0), grids=({t: g}, None, dims))
-    n, dt, first = 0, 0.0, 0
-    while dt < target_s:
-        tt = time.perf_counter()
-        mc.widom(0, first, 20000)
-        dt += time.perf_counter() - tt
-        first += 20000
-        n += 20000
+    chunk = 20000
+
+    def make(i):
+        mc = oracle.MC(spec, fw, [w.methane_component(spec)], params, kmax=(0, 0, 0), grids=({t: g}, None, dims))
+        mc.next_first = i * (1 << 30)
+        return mc
+
+    def step(mc):
+        mc.widom(0, mc.next_first, chunk)
+        mc.next_first += chunk
+        return chunk
+
+    _, counts, wall = parallel_timed(cores, make, step, target_s)
+    n = sum(counts)
     sites = n * params["k_first"]
-    return dict(value=sites / dt, unit="trial site energies/s", cores=1, kind="port",
-                sample=f"oracle Widom, CH4 in configuration 0, {n} insertions x {params['k_first']} trials in "
-                       f"{dt:.1f} s; insertions are independent, so C host cores give at most C times this")
+    return dict(value=sites / wall, unit="trial site energies/s", cores=cores, kind="port",
+                sample=f"oracle Widom, CH4 in configuration 0, {n} insertions x {params['k_first']} trials on "
+                       f"{cores} threads in {wall:.1f} s")
 
 
 def main():
@@ -918,6 +977,10 @@ def main():
     ap.add_argument("--mixture-pressure", type=float, default=1e5)
     ap.add_argument("--mixture-voxels", type=int, default=16, help="proposal voxels per unit-cell axis")
     ap.add_argument("--mixture-kfirst", type=int, default=2, help="first-bead trials of the learned mixture run")
+    ap.add_argument("--no-mixture-balance", action="store_true",
+                    help="one cycle per system and launch in the mixture (no balanced launches)")
+    ap.add_argument("--mixture-balance-overhead", type=float, default=256.0,
+                    help="balanced launches: a cycle costs its steps x (molecules + this)")
     ap.add_argument("--mixture-plain-same-k", action="store_true",
                     help="also run plain CBMC in the mixture with the learned run's first-bead trial count")
     ap.add_argument("--mixture-max-molecules", type=int, default=512)

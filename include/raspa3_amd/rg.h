@@ -398,6 +398,15 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
  * one wavefront per system with 1, 2 or 4 systems per workgroup; 3 runs a team of four wavefronts per system.  Every
  * shape computes the same trajectory bit for bit, so this only pins the speed (for reproducible timings). */
 rg_status rg_mc_set_launch_shape(rg_ctx* ctx, rg_mc* mc, int shape);
+/* Balanced launches (no counterpart in the reference, whose systems run one after another): with enable != 0 each
+ * of the 'cycles' launches of rg_mc_run gives every system at least one cycle and lets it run further whole cycles
+ * while their summed cost stays within the largest single-cycle cost over the systems at the start of the launch,
+ * a cycle's cost being its steps times (molecules + overhead_molecules).  Lightly loaded or small systems then do
+ * more cycles in the time the most loaded one takes for its first, instead of leaving their compute units idle; each
+ * system's trajectory is the same sequence of cycles as without balancing (only their number changes), and the
+ * samples of a launch go to the block of the launch's index among the sampled launches.  Rigid components only
+ * (RG_ERR_UNSUPPORTED with a flexible one); set it before the run it applies to. */
+rg_status rg_mc_set_balance(rg_ctx* ctx, rg_mc* mc, int enable, double overhead_molecules);
 rg_status rg_mc_state(rg_ctx* ctx, const rg_mc* mc, int system, rg_mc_state_t* out);
 /* Adsorbate atoms of a system (max(n_atoms) of the components slots per molecule; unused slots have type -1)
  * and the component of each molecule. */

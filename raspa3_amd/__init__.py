@@ -506,6 +506,12 @@ class Mc(_Handle):
         """-1: autotuned; 0, 1, 2: one wavefront per system, 1, 2 or 4 systems per workgroup; 3: four per system."""
         L.check(lib().rg_mc_set_launch_shape(self.ctx.h, self.h, int(shape)), self.ctx.h, "rg_mc_set_launch_shape")
 
+    def set_balance(self, enable=True, overhead_molecules=256.0):
+        """Balanced launches: every system runs at least one cycle per launch and further whole cycles while their
+        cost (steps x (molecules + overhead_molecules)) stays within the largest single-cycle cost of the launch."""
+        L.check(lib().rg_mc_set_balance(self.ctx.h, self.h, int(bool(enable)), float(overhead_molecules)),
+                self.ctx.h, "rg_mc_set_balance")
+
     def run(self, cycles, sample=False, graph=True):
         L.check(lib().rg_mc_run(self.ctx.h, self.h, int(cycles), int(sample), int(graph)), self.ctx.h, "rg_mc_run")
 

@@ -158,6 +158,7 @@ SIGNATURES.update({
                                C.POINTER(vp)]),
     "rg_mc_destroy": (C.c_int, [vp]),
     "rg_mc_set_launch_shape": (C.c_int, [vp, vp, C.c_int]),
+    "rg_mc_set_balance": (C.c_int, [vp, vp, C.c_int, C.c_double]),
     "rg_mc_set_proposal": (C.c_int, [vp, vp, C.c_int, ip, dp]),
     "rg_mc_set_component_proposal": (C.c_int, [vp, vp, C.c_int, C.c_int, ip, dp]),
     "rg_mc_run": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_int]),

@@ -1553,11 +1553,47 @@ __device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const Sy
   }
 }
 
+// The cost of one cycle of a system in a balanced launch: its steps times (molecules + the run's overhead constant),
+// a model of a step's work (the inter-molecular energies grow with the molecules, the grid and Ewald parts do not).
+__device__ __forceinline__ double balancedCycleCost(const McRun& R, int nmol)
+{
+  return double(max(R.minStepsPerCycle, nmol) * R.ncomp) * (double(nmol) + R.balanceOverhead);
+}
+
+// Before each balanced launch: the largest single-cycle cost over the systems, and the launch's sampled index.  One
+// workgroup; thread 0 writes the control record.
+__global__ __launch_bounds__(256) void k_mc_budget(const McRun* __restrict__ Rp, const McSys* __restrict__ Yarr,
+                                                   int nsys, McCtl* __restrict__ ctl, int sample)
+{
+  __shared__ double red[4];
+  const McRun& R = *Rp;
+  double m = 0.0;
+  for (int s = threadIdx.x; s < nsys; s += blockDim.x) m = fmax(m, balancedCycleCost(R, Yarr[s].nmol));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    for (int w = 1; w < int(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    ctl->budget = m;
+    if (sample)
+    {
+      ctl->sampledIndex = ctl->sampledCount;
+      ctl->sampledCount += 1;
+    }
+  }
+}
+
 // One wavefront per system, blockDim.x / 64 (1, 2 or 4) systems per workgroup: with few systems, one per workgroup
 // spreads them over all compute units.  Dynamic LDS: one WaveScratch per wavefront.
-template <bool CHAIN>
+// BAL: balanced launch (rigid components only): after its 'cycles' cycles a system keeps running whole cycles while
+// their summed cost stays within the budget k_mc_budget wrote, so lightly loaded systems do more cycles in the time
+// the most loaded one takes for its first; the samples of the launch go to the block of its sampled index.
+// Otherwise every system runs 'cycles' cycles and a sample goes to the block of the system's sampled-cycle count.
+template <bool CHAIN, bool BAL>
 __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
-                                              McSys* __restrict__ Yarr, int nsys, int cycles, int sample, int nw)
+                                              McSys* __restrict__ Yarr, int nsys, int cycles, int sample, int nw,
+                                              const McCtl* __restrict__ ctl)
 {
   extern __shared__ WaveScratch scratch[];
   // each wavefront's system descriptors, copied into LDS: the moves read their fields constantly, and LDS reads
@@ -1598,10 +1634,20 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
     L.trials[q] = Y.stat.trials[q];
     L.accepted[q] = Y.stat.accepted[q];
   }
-  for (int cyc = 0; cyc < cycles; ++cyc)
+  const double budget = BAL ? ctl->budget : 0.0;
+  double spent = 0.0;
+  for (int cyc = 0;; ++cyc)
   {
     // steps per cycle = max(20, N) times the number of components (src/raspakit/drivers/monte_carlo.cpp:301)
     const int nsteps = max(R.minStepsPerCycle, L.nmol) * R.ncomp;
+    if (cyc >= cycles)
+    {
+      // L.nmol is the same on every wavefront of a team, so the whole team leaves together
+      if (!BAL || cyc >= MC_MAX_BALANCED_CYCLES) break;
+      const double cost = balancedCycleCost(R, L.nmol);
+      if (spent + cost > budget) break;
+    }
+    if (BAL) spent += balancedCycleCost(R, L.nmol);
     for (int s = 0; s < nsteps; ++s)
     {
       gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w], T);
@@ -1617,7 +1663,8 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
       // block sums live in global memory, read and written by lane 0 of the team's first wavefront only
       if (wi == 0 && lane == 0)
       {
-        const int b = min(MC_BLOCKS - 1, int(L.cycle * MC_BLOCKS / max(1LL, R.productionCycles)));
+        const long long ci = BAL ? (long long)ctl->sampledIndex : L.cycle;
+        const int b = min(MC_BLOCKS - 1, int(ci * MC_BLOCKS / max(1LL, R.productionCycles)));
         McBlock& B = Yarr[sys].block[b];
         B.count += 1.0;
         B.U += L.U;
@@ -1733,7 +1780,8 @@ template __global__ void k_widom<false>(const McRun*, const SysDev*, const McSys
                                         double*, double*);
 template __global__ void k_widom<true>(const McRun*, const SysDev*, const McSys*, int, long long, int, double*,
                                        double*, double*);
-template __global__ void k_gcmc<false>(const McRun*, SysDev*, McSys*, int, int, int, int);
-template __global__ void k_gcmc<true>(const McRun*, SysDev*, McSys*, int, int, int, int);
+template __global__ void k_gcmc<false, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<false, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<true, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
 
 }  // namespace rg

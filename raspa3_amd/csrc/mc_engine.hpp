@@ -91,8 +91,22 @@ struct McRun
   double pTranslation, pRotation, pReinsertion;  // swap probability = 1 - all three, insertion/deletion 50/50
   double maxTranslation, maxRotation;
   long long productionCycles;
+  double balanceOverhead;  // balanced launches: a cycle's cost is its steps times (molecules + this)
   Comp comp[MC_MAXCOMP];
 };
+
+// Per-launch control of balanced Monte Carlo launches, written by k_mc_budget before each k_gcmc launch: every
+// system runs whole cycles while their summed cost stays within 'budget' (the largest single-cycle cost over the
+// systems at the start of the launch); 'sampledIndex' is the launch's index among the sampled launches, which picks
+// the block its samples go to.
+struct McCtl
+{
+  double budget;
+  int sampledIndex;
+  int sampledCount;
+};
+// at most this many cycles of one system per balanced launch
+#define MC_MAX_BALANCED_CYCLES 64
 
 // LDS of one wavefront in the Monte Carlo kernels: trial atom positions, Ewald eik tables and the grown molecule.
 // sites of one energy batch (K trial sets of up to MC_MAXRIGID - 1 sites), candidate pairs per flush, and force

@@ -10,8 +10,10 @@
 
 namespace rg
 {
-template <bool CHAIN>
-__global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample, int nw);
+template <bool CHAIN, bool BAL>
+__global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample, int nw,
+                       const McCtl* ctl);
+__global__ void k_mc_budget(const McRun* Rp, const McSys* Yarr, int nsys, McCtl* ctl, int sample);
 __global__ void k_widom_single(const McRun* Rp, const SysDev* Sp, const McSys* Yp, int comp, long long first,
                                int count, double* outLogW, double* outU, double* outEwald);
 template <bool CHAIN>
@@ -42,6 +44,9 @@ struct rg_mc
   int graphShape = -1;  // launch-shape candidate the graph was captured with
   int forcedShape = -1;  // rg_mc_set_launch_shape: fixed candidate, or -1 for the autotuner
   int computeUnits = 256;
+  McCtl* dCtl = nullptr;  // balanced launches: budget and sampled-launch index (k_mc_budget)
+  bool balance = false;   // rg_mc_set_balance
+  int graphBalance = -1;
 };
 
 namespace
@@ -203,11 +208,14 @@ rg_status rg_mc_create(rg_ctx* ctx, int n_systems, rg_system* const* systems, in
   if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dRun), sizeof(McRun));
   if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dS), sizeof(SysDev) * n_systems);
   if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dY), sizeof(McSys) * n_systems);
+  if (st == RG_OK) st = alloc(reinterpret_cast<void**>(&mc->dCtl), sizeof(McCtl));
   if (st != RG_OK)
   {
     rg_mc_destroy(mc);
     return st;
   }
+  const McCtl ctl0{};
+  RG_HIP(ctx, hipMemcpyAsync(mc->dCtl, &ctl0, sizeof(McCtl), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(mc->dRun, &mc->run, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(mc->dS, hS.data(), sizeof(SysDev) * n_systems, hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(mc->dY, mc->hY.data(), sizeof(McSys) * n_systems, hipMemcpyHostToDevice, ctx->stream));
@@ -357,6 +365,21 @@ rg_status rg_mc_set_launch_shape(rg_ctx* ctx, rg_mc* mc, int shape)
   return RG_OK;
 }
 
+rg_status rg_mc_set_balance(rg_ctx* ctx, rg_mc* mc, int enable, double overhead_molecules)
+{
+  if (!ctx || !mc || !std::isfinite(overhead_molecules) || overhead_molecules < 0.0)
+    return fail(ctx, RG_ERR_INVALID, "rg_mc_set_balance: invalid arguments");
+  RG_ENTER(ctx);
+  for (int c = 0; enable && c < mc->run.ncomp; ++c)
+    if (mc->run.comp[c].flexible)
+      return fail(ctx, RG_ERR_UNSUPPORTED, "rg_mc_set_balance: balanced launches are built for rigid molecules only");
+  mc->balance = enable != 0;
+  mc->run.balanceOverhead = overhead_molecules;
+  RG_HIP(ctx, hipMemcpyAsync(mc->dRun, &mc->run, sizeof(McRun), hipMemcpyHostToDevice, ctx->stream));
+  RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return RG_OK;
+}
+
 rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_graph)
 {
   if (!ctx || !mc || cycles < 0) return fail(ctx, RG_ERR_INVALID, "rg_mc_run: invalid arguments");
@@ -370,12 +393,14 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   static const int kWpb[4] = {1, 2, 4, 1};
   bool chain = false;
   for (int c = 0; c < mc->run.ncomp; ++c) chain = chain || mc->run.comp[c].flexible;
+  const bool bal = mc->balance && !chain;
   auto launch = [&](int cand)
   {
     const int nw = kNw[cand], wpb = kWpb[cand];
-    hipLaunchKernelGGL(chain ? k_gcmc<true> : k_gcmc<false>, dim3((nsys + wpb - 1) / wpb), dim3(64 * nw * wpb),
-                       nw * wpb * sizeof(WaveScratch), ctx->stream,
-                       mc->dRun, mc->dS, mc->dY, nsys, 1, sample, nw);
+    if (bal) hipLaunchKernelGGL(k_mc_budget, dim3(1), dim3(256), 0, ctx->stream, mc->dRun, mc->dY, nsys, mc->dCtl, sample);
+    auto kernel = chain ? k_gcmc<true, false> : bal ? k_gcmc<false, true> : k_gcmc<false, false>;
+    hipLaunchKernelGGL(kernel, dim3((nsys + wpb - 1) / wpb), dim3(64 * nw * wpb), nw * wpb * sizeof(WaveScratch),
+                       ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample, nw, static_cast<const McCtl*>(mc->dCtl));
     return hipGetLastError();
   };
   Autotuner& tuner = Autotuner::instance();
@@ -386,7 +411,7 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   int bucket = 0;
   while ((2ll << bucket) <= atoms) ++bucket;
   const std::string key = "k_gcmc/systems=" + std::to_string(nsys) + "/cu=" + std::to_string(mc->computeUnits) +
-                          "/atoms>=" + std::to_string(atoms > 0 ? (1ll << bucket) : 0);
+                          "/atoms>=" + std::to_string(atoms > 0 ? (1ll << bucket) : 0) + (bal ? "/balanced" : "");
   int done = 0;
   for (int cand; mc->forcedShape < 0 && done < cycles && (cand = tuner.next(key, 4, 2)) >= 0; ++done)
   {
@@ -402,8 +427,9 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   }
   else if (done < cycles)
   {
-    // one MC cycle of every system captured once as a graph, replayed per cycle
-    if (!mc->graphExec || mc->graphSample != sample || mc->graphShape != shape)
+    // one MC cycle of every system (with its budget kernel when balanced) captured once as a graph, replayed per
+    // cycle
+    if (!mc->graphExec || mc->graphSample != sample || mc->graphShape != shape || mc->graphBalance != int(bal))
     {
       if (mc->graphExec) (void)hipGraphExecDestroy(mc->graphExec);
       if (mc->graph) (void)hipGraphDestroy(mc->graph);
@@ -417,6 +443,7 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
       RG_HIP(ctx, hipGraphInstantiate(&mc->graphExec, mc->graph, nullptr, nullptr, 0));
       mc->graphSample = sample;
       mc->graphShape = shape;
+      mc->graphBalance = int(bal);
     }
     for (int c = done; c < cycles; ++c) RG_HIP(ctx, hipGraphLaunch(mc->graphExec, ctx->stream));
   }

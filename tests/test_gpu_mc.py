@@ -431,6 +431,81 @@ def test_gcmc_uneven_batch_replays_oracle(gpu_ctx):
     mc.close()
 
 
+def replay_to_steps(om, target, sample):
+    """Runs the oracle's cycles one at a time until its step counter reaches target (a balanced launch runs whole
+    cycles, so the counter must land on it exactly)."""
+    while om.state()["steps"] < target:
+        om.run(1, sample=sample)
+    assert om.state()["steps"] == target
+
+
+def test_balanced_launches_replay_oracle(gpu_ctx):
+    """Balanced launches (rg_mc_set_balance) over uneven systems: CO2 (Ewald kmax 6) at 1e5 Pa in charged ITQ-29
+    3x3x3, 3x3x4 and 4x4x4 in one launch per cycle.  Each system replays the oracle's trajectory: the oracle runs the
+    same cycles one after another, matched launch by launch through the step counter; the samples of sampled launch i
+    land in block i with exactly the oracle's sums over that launch's cycles; and the less loaded systems run more
+    cycles per launch than the most loaded one."""
+    print("case: balanced launches, CO2 in ITQ-29 3x3x3 / 3x3x4 / 4x4x4", flush=True)
+    spec = w.forcefield_zeolite()
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    fw3, unit, ff, f3, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    fws = [fw3] + [w.framework_itq29(spec, c)[0] for c in ((3, 3, 4), (4, 4, 4))]
+    fos = [f3] + [ra.Framework(gpu_ctx, fw) for fw in fws[1:]]
+    comp = w.co2_reference(spec)
+    params = w.mc_params(pressure_pa=1e5, seed=29, max_molecules=160, production_cycles=4)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, True, framework=fo, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                         capacity=8 * 160) for fw, fo in zip(fws, fos)]
+    mc = ra.Mc(gpu_ctx, systems, [comp], params)
+    # cost = steps x molecules: the 4x4x4 system (2.4 times the 3x3x3 volume) sets the budget once loaded (while
+    # filling, systems of any size gain about the same number of molecules per cycle, so loadings part only later)
+    mc.set_balance(True, 0.0)
+    mc.run(60, sample=False, graph=True)
+    eq_steps = [mc.state(s)["steps"] for s in range(3)]
+    launch_steps = []
+    for _ in range(4):
+        mc.run(1, sample=True, graph=True)
+        launch_steps.append([mc.state(s)["steps"] for s in range(3)])
+    cycles = [mc.state(s)["cycles"] for s in range(3)]
+    print("sampled cycles per system over 4 launches:", cycles, "molecules:",
+          [mc.state(s)["n_molecules"] for s in range(3)], flush=True)
+    for sidx, fw in enumerate(fws):
+        om = oracle.MC(spec, fw, [comp], params, kmax=(6, 6, 6), system=sidx, grids=(host, hcoul, dims, unit))
+        replay_to_steps(om, eq_steps[sidx], False)
+        g = mc.state(sidx)
+        for i in range(4):
+            before = om.state()["blocks"].sum(0)
+            replay_to_steps(om, launch_steps[i][sidx], True)
+            d = om.state()["blocks"].sum(0) - before
+            assert g["block_count"][i] == d[0] and g["block_n"][i][0] == d[2] and g["block_nn"][i][0][0] == d[6]
+            assert close(g["block_energy"][i], d[1], rel=1e-8, abs_=1e-6)
+            assert close(g["block_energy_n"][i][0], d[4], rel=1e-8, abs_=1e-6)
+        assert g["block_count"][4] == 0
+        o = om.state()
+        assert g["n_molecules"] == o["n_molecules"] and g["cycles"] == o["cycles"]
+        assert list(g["trials"]) == list(o["trials"]) and list(g["accepted"]) == list(o["accepted"])
+        assert close(g["energy"], o["energy"], rel=1e-8, abs_=1e-6)
+        gp, gt, gc = mc.adsorbates(sidx)
+        op, ot, oc = om.adsorbates()
+        assert np.array_equal(gt, ot) and np.array_equal(gc, oc)
+        assert np.allclose(gp, op, rtol=0, atol=1e-8)
+    assert min(cycles) >= 4 and max(cycles) > 4
+    tot = mc.total_energy()
+    for sidx in range(3):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
+
+
+def test_balanced_launches_are_refused_for_chains(gpu_ctx):
+    """The balanced kernel is built for rigid molecules only."""
+    spec, fw, unit, ff, f, dims, grids, host = hexane_itq29(gpu_ctx)
+    s = ra.System(gpu_ctx, ff, fw.cell, True, framework=f, grids=grids, kmax=(0, 0, 0), capacity=8 * 64)
+    mc = ra.Mc(gpu_ctx, [s], [w.hexane_component(spec, 0.3)], w.mc_params(max_molecules=64))
+    with pytest.raises(ra.RGError, match="rigid"):
+        mc.set_balance(True)
+    mc.close()
+
+
 @pytest.mark.parametrize("case", ["co2", "hexane"])
 def test_launch_shapes_give_identical_trajectories(gpu_ctx, case):
     """Every launch shape (one wavefront per system with 1, 2 or 4 systems per workgroup, or a team of four
