@@ -951,6 +951,8 @@ def main():
     ap.add_argument("--gcmc-only", default="", metavar="NAMES",
                     help="measure the headline grid build and only the GCMC comparisons named (itq29, mfi, channels; "
                          "name@pressure_pa sets the pressure)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="measure the headline grid build and the proposal-network training only")
     ap.add_argument("--quintic-only", action="store_true",
                     help="measure the headline grid build and the triquintic grids of the same workload only")
     ap.add_argument("--widom-insertions", type=int, default=1 << 20)
@@ -1013,6 +1015,8 @@ def main():
     ctx = ra.Context(0)
     value, ms_per_step, npts, roof, g1 = bench_grids(ctx, args)
     extra = {"grid_check": g1["grid_check"], "grid_split_error": g1["split_error"]}
+    if args.mlp_only:
+        extra["mlp"] = bench_mlp(ctx, g1, args)
     if args.quintic_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
     if args.gcmc_only:
@@ -1022,7 +1026,7 @@ def main():
                                                pressure=float(pa) if pa else 1e5)
     elif args.mixture_only:
         extra["gcmc_mixture"] = bench_gcmc_mixture(ctx, args)
-    elif not args.grids_only and not args.quintic_only:
+    elif not args.grids_only and not args.quintic_only and not args.mlp_only:
         extra["grids_triquintic"] = bench_grids_quintic(ctx, g1, args)
         extra["trial_energies"] = bench_trial_energies(ctx, g1, args)
         extra["mlp"] = bench_mlp(ctx, g1, args)

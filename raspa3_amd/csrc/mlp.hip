@@ -8,9 +8,10 @@
 // Weights are kept as FP32 masters (updated by Adam) and rounded to BF16 for the products; activations are stored
 // in BF16 for the next product and pre-activations in FP32 for the backward pass.
 //
-// GEMM tile: 64 x 64 outputs per workgroup, four wavefronts of 32 x 32 (2 x 2 MFMA 16 x 16 x 32 blocks); the whole K
-// extent of a stage (up to 256) is staged through LDS with the K index contiguous, so every MFMA operand is one
-// 16-byte LDS read, and all of a stage's global loads are in flight together.
+// A training step is seven kernels: the fused forward pass (features, three layers and the output for 16 rows per
+// workgroup, activations in LDS), the loss, the fused backward pass down to the first layer's pre-activation
+// gradients, the bias and output-weight gradients as row sums, three weight-gradient products (64 x 64 tiles,
+// split K, operands stored feature-major so every MFMA operand is one 16-byte LDS read), and Adam.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -32,26 +33,16 @@ constexpr int WIDTH = 256;
 constexpr int NFREQ = 32;
 constexpr int NIN = 6 * NFREQ;  // 192
 
-enum Epilogue
-{
-  EPI_BIAS_SILU = 0,  // Z = acc + bias (FP32 out), H = SiLU(Z) (BF16 out)
-  EPI_STORE_F32 = 1,  // C = acc (FP32 out)
-  EPI_DSILU = 2,      // dZ = acc * SiLU'(Zprev) (BF16 out), column sums of dZ added into outF (bias gradient)
-  EPI_ATOMIC_F32 = 3  // C += acc (FP32, split-K partial products of the weight gradients)
-};
-
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-// C (M x N) = op(A) (M x K) * op(B) (K x N).  TA: A is stored K x M; TB: B is stored N x K.  All row-major.
-// Workgroup (x, y, z) computes the 64 x 64 tile (y, x) over k in [z Kc, min(K, (z + 1) Kc)).  K is taken in stages
-// of up to KS = 256: every global load of a stage is issued before any of them is written to LDS, so a stage costs
-// one memory latency (the network's K extents, 192 and 256, and the weight-gradient chunks are one stage each).
+// Weight gradient C (M x N) += A B^T over k, with A (M x K) and B (N x K) row-major (k contiguous: the layer inputs
+// and pre-activation gradients stored feature-major).  Workgroup (x, y, z) computes the 64 x 64 tile (y, x) over k in
+// [z Kc, min(K, (z + 1) Kc)) and adds it atomically (split K).  K is taken in stages of up to KS = 256: every global
+// load of a stage is issued before any of them is written to LDS, so a stage costs one memory latency.
 constexpr int KS = 256;
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const __bf16* __restrict__ A, int lda,
-                                              const __bf16* __restrict__ B, int ldb, const float* __restrict__ bias,
-                                              const float* __restrict__ Zprev, float* __restrict__ outF,
-                                              __bf16* __restrict__ outH)
+__global__ __launch_bounds__(256) void k_weight_grad(int M, int N, int K, int Kc, const __bf16* __restrict__ A,
+                                                     int lda, const __bf16* __restrict__ B, int ldb,
+                                                     float* __restrict__ C)
 {
   __shared__ __attribute__((aligned(16))) __bf16 As[64][KS + 8];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[64][KS + 8];
@@ -66,7 +57,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
   {
     const int kk = min(KS, kEnd - k0);  // a multiple of 32
     const int nv = 8 * kk;               // 8-element vectors per operand tile (64 rows x kk)
-    const int rowv = kk / 8;             // vectors along k per row of a k-contiguous operand
+    const int rowv = kk / 8;             // vectors along k per row
     bf16x8 va[KS / 32], vb[KS / 32];
 #pragma unroll
     for (int q = 0; q < KS / 32; ++q)
@@ -74,10 +65,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
       const int v = t + 256 * q;
       if (v < nv)
       {
-        va[q] = !TA ? *reinterpret_cast<const bf16x8*>(A + size_t(m0 + v / rowv) * lda + k0 + (v % rowv) * 8)
-                    : *reinterpret_cast<const bf16x8*>(A + size_t(k0 + (v >> 3)) * lda + m0 + (v & 7) * 8);
-        vb[q] = !TB ? *reinterpret_cast<const bf16x8*>(B + size_t(k0 + (v >> 3)) * ldb + n0 + (v & 7) * 8)
-                    : *reinterpret_cast<const bf16x8*>(B + size_t(n0 + v / rowv) * ldb + k0 + (v % rowv) * 8);
+        va[q] = *reinterpret_cast<const bf16x8*>(A + size_t(m0 + v / rowv) * lda + k0 + (v % rowv) * 8);
+        vb[q] = *reinterpret_cast<const bf16x8*>(B + size_t(n0 + v / rowv) * ldb + k0 + (v % rowv) * 8);
       }
     }
 #pragma unroll
@@ -86,20 +75,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
       const int v = t + 256 * q;
       if (v < nv)
       {
-        if (!TA)
-          *reinterpret_cast<bf16x8*>(&As[v / rowv][(v % rowv) * 8]) = va[q];
-        else
-        {
-#pragma unroll
-          for (int i = 0; i < 8; ++i) As[(v & 7) * 8 + i][v >> 3] = va[q][i];
-        }
-        if (!TB)
-        {
-#pragma unroll
-          for (int i = 0; i < 8; ++i) Bs[(v & 7) * 8 + i][v >> 3] = vb[q][i];
-        }
-        else
-          *reinterpret_cast<bf16x8*>(&Bs[v / rowv][(v % rowv) * 8]) = vb[q];
+        *reinterpret_cast<bf16x8*>(&As[v / rowv][(v % rowv) * 8]) = va[q];
+        *reinterpret_cast<bf16x8*>(&Bs[v / rowv][(v % rowv) * 8]) = vb[q];
       }
     }
     __syncthreads();
@@ -119,226 +96,19 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, int Kc, const
     }
     __syncthreads();
   }
-  float csum[2] = {0.f, 0.f};  // EPI_DSILU: this lane's share of the column sums of dZ
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-      {
-        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
-        const int n = n0 + wn + 16 * j + (lane & 15);
-        const size_t o = size_t(m) * N + n;
-        const float v = acc[i][j][r];
-        if (EPI == EPI_BIAS_SILU)
-        {
-          const float z = v + bias[n];
-          outF[o] = z;
-          outH[o] = (__bf16)(z * sigmoidf(z));
-        }
-        else if (EPI == EPI_STORE_F32)
-          outF[o] = v;
-        else if (EPI == EPI_ATOMIC_F32)
-          atomicAdd(&outF[o], v);
-        else
-        {
-          const float z = Zprev[o];
-          const float s = sigmoidf(z);
-          const __bf16 h = (__bf16)(v * s * (1.0f + z * (1.0f - s)));
-          outH[o] = h;
-          csum[j] += float(h);
-        }
-      }
-  if (EPI == EPI_DSILU)
-  {
-    // bias gradient: the tile's column sums (lanes l, l + 16, l + 32, l + 48 hold column l), added into outF[n]
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-    {
-      float cs = csum[j];
-      cs += __shfl_xor(cs, 16, 64);
-      cs += __shfl_xor(cs, 32, 64);
-      if (lane < 16) atomicAdd(&outF[n0 + wn + 16 * j + lane], cs);
-    }
-  }
+        atomicAdd(&C[size_t(m0 + wm + 16 * i + 4 * (lane >> 4) + r) * N + n0 + wn + 16 * j + (lane & 15)],
+                  acc[i][j][r]);
 }
 
-// Fourier features of fractional coordinates
-__global__ void k_features(int n, const double* __restrict__ frac, __bf16* __restrict__ X)
-{
-  const int i = blockIdx.x * blockDim.y + threadIdx.y;
-  if (i >= n) return;
-  for (int q = threadIdx.x; q < NIN; q += blockDim.x)
-  {
-    // the phase is reduced to one period in FP64 (m s - floor(m s)), then evaluated in FP32, whose 1e-7 accuracy
-    // is far below the BF16 resolution of the stored feature
-    const int a = q / (2 * NFREQ), m = (q % (2 * NFREQ)) / 2 + 1, c = q & 1;
-    const double ms = m * frac[3 * i + a];
-    const float ph = float(2.0 * M_PI * (ms - floor(ms)));
-    float sn, cs;
-    sincosf(ph, &sn, &cs);
-    X[size_t(i) * NIN + q] = (__bf16)(c ? cs : sn);
-  }
-}
-
-// f_i = H3_i . w4 + b4 (one wavefront per row)
-__global__ void k_out_forward(int n, const __bf16* __restrict__ H, const __bf16* __restrict__ w4,
-                              const float* __restrict__ b4, float* __restrict__ f)
-{
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  float s = 0.f;
-  for (int j = lane; j < WIDTH; j += 64) s += float(H[size_t(i) * WIDTH + j]) * float(w4[j]);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) f[i] = s + b4[0];
-}
-
-// One workgroup: p = softmax(-beta U), q = softmax(f), df = q - p, loss = -sum p log q
-__global__ __launch_bounds__(1024) void k_loss(int n, const float* __restrict__ f, const double* __restrict__ U,
-                                               double beta, float* __restrict__ df, double* __restrict__ loss)
-{
-  __shared__ double red[16];
-  const int t = threadIdx.x;
-  auto blockMax = [&](double v)
-  {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if ((t & 63) == 0) red[t >> 6] = v;
-    __syncthreads();
-    double r = -INFINITY;
-    for (int i = 0; i < int(blockDim.x >> 6); ++i) r = fmax(r, red[i]);
-    __syncthreads();
-    return r;
-  };
-  auto blockSum = [&](double v)
-  {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((t & 63) == 0) red[t >> 6] = v;
-    __syncthreads();
-    double r = 0.0;
-    for (int i = 0; i < int(blockDim.x >> 6); ++i) r += red[i];
-    __syncthreads();
-    return r;
-  };
-  double mf = -INFINITY, mp = -INFINITY;
-  for (int i = t; i < n; i += blockDim.x)
-  {
-    mf = fmax(mf, double(f[i]));
-    mp = fmax(mp, -beta * U[i]);
-  }
-  mf = blockMax(mf);
-  mp = blockMax(mp);
-  double sf = 0.0, sp = 0.0;
-  for (int i = t; i < n; i += blockDim.x)
-  {
-    sf += exp(double(f[i]) - mf);
-    sp += exp(-beta * U[i] - mp);
-  }
-  sf = blockSum(sf);
-  sp = blockSum(sp);
-  const double lse = mf + log(sf);
-  double l = 0.0;
-  for (int i = t; i < n; i += blockDim.x)
-  {
-    const double p = exp(-beta * U[i] - mp) / sp;
-    const double q = exp(double(f[i]) - lse);
-    df[i] = float(q - p);
-    l -= p * (double(f[i]) - lse);
-  }
-  l = blockSum(l);
-  if (t == 0)
-  {
-    loss[0] = l;
-    loss[1] += l;
-  }
-}
-
-// Output layer backward over a chunk of OB_ROWS rows per workgroup, one thread per hidden unit j:
-// dw4_j += sum_i H3_ij df_i, db4 += sum_i df_i, dZ3_ij = df_i w4_j SiLU'(Z3_ij) (BF16), db3_j += sum_i dZ3_ij.
-constexpr int OB_ROWS = 32;
-__global__ void k_out_backward(int n, const __bf16* __restrict__ H, const float* __restrict__ Z,
-                               const __bf16* __restrict__ w4, const float* __restrict__ df, float* __restrict__ gw4,
-                               float* __restrict__ gb4, __bf16* __restrict__ dZ, float* __restrict__ gb3)
-{
-  const int j = threadIdx.x;  // blockDim = WIDTH
-  const int i0 = blockIdx.x * OB_ROWS;
-  const int i1 = min(n, i0 + OB_ROWS);
-  float g = 0.f, gb = 0.f, cs = 0.f;
-  const float wj = float(w4[j]);
-#pragma unroll 4
-  for (int i = i0; i < i1; ++i)
-  {
-    const size_t o = size_t(i) * WIDTH + j;
-    const float d = df[i];
-    g += float(H[o]) * d;
-    gb += d;
-    const float z = Z[o], s = sigmoidf(z);
-    const __bf16 h = (__bf16)(d * wj * s * (1.0f + z * (1.0f - s)));
-    dZ[o] = h;
-    cs += float(h);
-  }
-  atomicAdd(&gw4[j], g);
-  atomicAdd(&gb3[j], cs);
-  if (j == 0) atomicAdd(gb4, gb);
-}
-
-__global__ void k_zero(float* p, int n)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.f;
-}
-
-// Adam (Kingma & Ba, 2015) on the FP32 masters, then the BF16 copies used by the products
-__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, __bf16* __restrict__ wb, float lr, float b1, float b2, float eps,
-                       float c1, float c2)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
-  const float mi = b1 * m[i] + (1.f - b1) * gi;
-  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-  m[i] = mi;
-  v[i] = vi;
-  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
-  w[i] = wi;
-  if (wb) wb[i] = (__bf16)wi;
-}
-
-// minibatch rows gathered from the device-resident training set through a permutation
-__global__ void k_gather(int n, const int* __restrict__ perm, const double* __restrict__ frac,
-                         const double* __restrict__ U, double* __restrict__ bfrac, double* __restrict__ bU)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int q = perm[i];
-  bfrac[3 * i] = frac[3 * size_t(q)];
-  bfrac[3 * i + 1] = frac[3 * size_t(q) + 1];
-  bfrac[3 * i + 2] = frac[3 * size_t(q) + 2];
-  bU[i] = U[q];
-}
-
-__global__ void k_zero_d(double* p, int n)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
-}
-
-__global__ void k_to_bf16(int n, const float* __restrict__ a, __bf16* __restrict__ b)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (__bf16)a[i];
-}
-
-}  // namespace
-}  // namespace rg
-
-using namespace rg;
-
-// parameter layout in one FP32 array: W1 (192x256), b1, W2 (256x256), b2, W3 (256x256), b3, w4 (256), b4
-namespace
-{
+// Parameter layout in one FP32 array: W1 (192 x 256), b1, W2 (256 x 256), b2, W3 (256 x 256), b3, w4 (256), b4;
+// each W stored in x out, row-major.  The fused forward pass reads transposed BF16 copies (out x in, row-major) of
+// W1..W3, laid out one after another.
 constexpr size_t OFF_W1 = 0;
 constexpr size_t OFF_B1 = OFF_W1 + size_t(NIN) * WIDTH;
 constexpr size_t OFF_W2 = OFF_B1 + WIDTH;
@@ -348,7 +118,442 @@ constexpr size_t OFF_B3 = OFF_W3 + size_t(WIDTH) * WIDTH;
 constexpr size_t OFF_W4 = OFF_B3 + WIDTH;
 constexpr size_t OFF_B4 = OFF_W4 + WIDTH;
 constexpr size_t NPARAM = OFF_B4 + 1;
+constexpr size_t OFF_T1 = 0;
+constexpr size_t OFF_T2 = OFF_T1 + size_t(WIDTH) * NIN;
+constexpr size_t OFF_T3 = OFF_T2 + size_t(WIDTH) * WIDTH;
+constexpr size_t NTRANS = OFF_T3 + size_t(WIDTH) * WIDTH;
+
+// index of parameter i in the transposed copies, or -1 for biases and w4
+__device__ __forceinline__ long long transposedIndex(size_t i)
+{
+  if (i < OFF_B1) return (long long)(OFF_T1 + (i % WIDTH) * NIN + i / WIDTH);
+  if (i >= OFF_W2 && i < OFF_B2) return (long long)(OFF_T2 + ((i - OFF_W2) % WIDTH) * WIDTH + (i - OFF_W2) / WIDTH);
+  if (i >= OFF_W3 && i < OFF_B3) return (long long)(OFF_T3 + ((i - OFF_W3) % WIDTH) * WIDTH + (i - OFF_W3) / WIDTH);
+  return -1;
+}
+
+// Fused kernels: one workgroup of four wavefronts takes FROWS batch rows through the network.  Activations stay in
+// LDS (row-major, k contiguous, row stride LDSW); every layer is computed in transposed form, Z^T = W^T H^T
+// (forward) and dH^T = W dZ^T (backward), so that the A operand is rows of a weight matrix read straight from global
+// memory (shared by all workgroups through the L2) and the B operand is rows of the LDS tile.  Wavefront w owns
+// output features 64 w .. 64 w + 63 (four 16-row A blocks) for all FROWS rows (RB 16-column B blocks); a lane's
+// four results per block are four consecutive features of one row, written to LDS as one 8-byte store.  16 rows per
+// workgroup spread a minibatch of 4096 over 256 workgroups, one per compute unit: the per-row work of the epilogues
+// (SiLU, stores) then runs on every SIMD of the chip rather than on a quarter of them.
+constexpr int FROWS = 16;
+constexpr int RB = FROWS / 16;
+constexpr int LDSW = WIDTH + 8;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// acc[i][j] += Wrows(64 w + 16 i ..) x tile rows (16 j ..) over k in [0, K): Wrows row-major with row stride K.
+// All of the wavefront's A fragments (K / 32 steps x 4 blocks, 16 B each per lane) are loaded before the first
+// product, so a layer waits for one memory latency rather than one per K step.
+template <int K>
+__device__ __forceinline__ void tileProduct(const __bf16* __restrict__ Wrows, const __bf16 (*tile)[LDSW], int w,
+                                            int lane, f32x4 (&acc)[4][RB])
+{
+  constexpr int NS = K / 32;
+  const int kq = 8 * (lane >> 4);
+  const __bf16* arow = Wrows + size_t(64 * w + (lane & 15)) * K + kq;
+  bf16x8 a[NS][4];
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[k][i] = *reinterpret_cast<const bf16x8*>(arow + size_t(16 * i) * K + 32 * k);
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+  {
+    bf16x8 b[RB];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&tile[16 * j + (lane & 15)][32 * k + kq]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[k][i], b[j], acc[i][j], 0, 0, 0);
+  }
+}
+
+// Writes nf features [f0, f0 + nf) of an LDS tile feature-major to global memory (out[f * ld + row0 + row]); a lane
+// takes row lane % FROWS of feature lane / FROWS + (64 / FROWS) k, so each store covers FROWS consecutive values of
+// 64 / FROWS features.
+__device__ __forceinline__ void storeFeatureMajor(const __bf16 (*tile)[LDSW], int f0, int nf, __bf16* __restrict__ out,
+                                                  int ld, int row0, int lane)
+{
+  constexpr int FPI = 64 / FROWS;
+  const int row = lane % FROWS;
+#pragma unroll 4
+  for (int f = f0 + lane / FROWS; f < f0 + nf; f += FPI) out[size_t(f) * ld + row0 + row] = tile[row][f];
+}
+
+// Forward pass of FROWS rows per workgroup: Fourier features (from frac, or gathered through perm with the energies
+// copied to bU), three SiLU layers and the output f = H3 . w4 + b4.  TRAIN also stores the pre-activations Z1..Z3
+// (row-major FP32) and the layer outputs X, H1, H2, H3 feature-major (ld = ldt) for the parameter gradients.
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void k_mlp_forward(const double* __restrict__ frac, const int* __restrict__ perm,
+                                                     const double* __restrict__ Uall, double* __restrict__ bU,
+                                                     const __bf16* __restrict__ wt, const __bf16* __restrict__ wb,
+                                                     const float* __restrict__ w, float* __restrict__ Z1,
+                                                     float* __restrict__ Z2, float* __restrict__ Z3, __bf16* __restrict__ Xt,
+                                                     __bf16* __restrict__ H1t, __bf16* __restrict__ H2t,
+                                                     __bf16* __restrict__ H3t, int ldt, float* __restrict__ f)
+{
+  __shared__ __attribute__((aligned(16))) __bf16 bufA[FROWS][LDSW];
+  __shared__ __attribute__((aligned(16))) __bf16 bufB[FROWS][LDSW];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int row0 = blockIdx.x * FROWS;
+  // features: thread (row, axis) evaluates sin and cos of 2 pi m s for m = 1..32 by the angle-addition recurrence in
+  // FP64 from one FP64 sincos (exact far below the BF16 resolution of the stored feature)
+  if (t < 3 * FROWS)
+  {
+    const int row = t & (FROWS - 1), ax = t / FROWS;
+    const int gi = row0 + row;
+    const int src = perm ? perm[gi] : gi;
+    if (TRAIN && perm && ax == 0) bU[gi] = Uall[src];
+    double s1, c1;
+    sincos(2.0 * M_PI * frac[3 * size_t(src) + ax], &s1, &c1);
+    double sp = 0.0, cp = 1.0, sm = s1, cm = c1;
+#pragma unroll
+    for (int m8 = 0; m8 < NFREQ / 4; ++m8)
+    {
+      bf16x8 v;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+      {
+        v[2 * q] = (__bf16)sm;
+        v[2 * q + 1] = (__bf16)cm;
+        const double sn = 2.0 * c1 * sm - sp, cn = 2.0 * c1 * cm - cp;
+        sp = sm;
+        cp = cm;
+        sm = sn;
+        cm = cn;
+      }
+      *reinterpret_cast<bf16x8*>(&bufA[row][2 * NFREQ * ax + 8 * m8]) = v;
+    }
+  }
+  __syncthreads();
+  if (TRAIN) storeFeatureMajor(bufA, 48 * wv, 48, Xt, ldt, row0, lane);
+  // three layers: A operand = transposed weights, B operand = the LDS tile of the layer input
+  __bf16 (*in)[LDSW] = bufA;
+  __bf16 (*out)[LDSW] = bufB;
+#pragma unroll 1
+  for (int l = 0; l < 3; ++l)
+  {
+    f32x4 acc[4][RB];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < RB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (l == 0)
+      tileProduct<NIN>(wt + OFF_T1, in, wv, lane, acc);
+    else
+      tileProduct<WIDTH>(wt + (l == 1 ? OFF_T2 : OFF_T3), in, wv, lane, acc);
+    const float* bias = w + (l == 0 ? OFF_B1 : l == 1 ? OFF_B2 : OFF_B3);
+    float* Z = l == 0 ? Z1 : l == 1 ? Z2 : Z3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+      const int o0 = 64 * wv + 16 * i + 4 * (lane >> 4);
+      const float4 bv = *reinterpret_cast<const float4*>(bias + o0);
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+      {
+        const int row = 16 * j + (lane & 15);
+        const float z0 = acc[i][j][0] + bv.x, z1 = acc[i][j][1] + bv.y, z2 = acc[i][j][2] + bv.z,
+                    z3 = acc[i][j][3] + bv.w;
+        if (TRAIN) *reinterpret_cast<float4*>(Z + size_t(row0 + row) * WIDTH + o0) = float4{z0, z1, z2, z3};
+        *reinterpret_cast<bf16x4*>(&out[row][o0]) =
+            bf16x4{(__bf16)(z0 * sigmoidf(z0)), (__bf16)(z1 * sigmoidf(z1)), (__bf16)(z2 * sigmoidf(z2)),
+                   (__bf16)(z3 * sigmoidf(z3))};
+      }
+    }
+    __syncthreads();
+    if (TRAIN) storeFeatureMajor(out, 64 * wv, 64, l == 0 ? H1t : l == 1 ? H2t : H3t, ldt, row0, lane);
+    __bf16 (*nin)[LDSW] = out;
+    out = in;
+    in = nin;
+  }
+  // output: f = H3 . w4 + b4, TPR consecutive threads per row over WIDTH / TPR features each
+  {
+    constexpr int TPR = 256 / FROWS, FPT = WIDTH / TPR;
+    const int row = t / TPR, part = t % TPR;
+    float s = 0.f;
+#pragma unroll 8
+    for (int k = FPT * part; k < FPT * part + FPT; ++k) s += float(in[row][k]) * float(wb[OFF_W4 + k]);
+#pragma unroll
+    for (int o = 1; o < TPR; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (part == 0) f[row0 + row] = s + w[OFF_B4];
+  }
+}
+
+// One workgroup: p = softmax(-beta U), q = softmax(f), df = q - p, loss = -sum p log q.  The maxima and the
+// exponent arguments are formed in FP64, the exponentials evaluated in FP32 (df is FP32), the sums accumulated in FP64.
+// Batches of up to LOSS_CACHE * LOSS_THREADS rows are read once into registers; larger ones are re-read per pass.
+constexpr int LOSS_THREADS = 1024, LOSS_CACHE = 8;
+__global__ __launch_bounds__(LOSS_THREADS) void k_loss(int n, const float* __restrict__ f, const double* __restrict__ U,
+                                               double beta, float* __restrict__ df, double* __restrict__ loss)
+{
+  __shared__ double red[2][16];
+  const int t = threadIdx.x, nt = blockDim.x;
+  auto reduce2 = [&](double& a, double& b, bool isMax)
+  {
+    for (int o = 32; o > 0; o >>= 1)
+    {
+      const double x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64);
+      a = isMax ? fmax(a, x) : a + x;
+      b = isMax ? fmax(b, y) : b + y;
+    }
+    if ((t & 63) == 0)
+    {
+      red[0][t >> 6] = a;
+      red[1][t >> 6] = b;
+    }
+    __syncthreads();
+    a = red[0][0];
+    b = red[1][0];
+    for (int i = 1; i < (nt >> 6); ++i)
+    {
+      a = isMax ? fmax(a, red[0][i]) : a + red[0][i];
+      b = isMax ? fmax(b, red[1][i]) : b + red[1][i];
+    }
+    __syncthreads();
+  };
+  double mf = -INFINITY, mp = -INFINITY, sf = 0.0, sp = 0.0, l = 0.0, unused = 0.0;
+  if (n <= LOSS_CACHE * nt)
+  {
+    float fc[LOSS_CACHE];
+    double uc[LOSS_CACHE];  // -beta U
+#pragma unroll
+    for (int k = 0; k < LOSS_CACHE; ++k)
+    {
+      const int i = t + k * nt;
+      fc[k] = i < n ? f[i] : 0.f;
+      uc[k] = i < n ? -beta * U[i] : 0.0;
+      if (i < n)
+      {
+        mf = fmax(mf, double(fc[k]));
+        mp = fmax(mp, uc[k]);
+      }
+    }
+    reduce2(mf, mp, true);
+#pragma unroll
+    for (int k = 0; k < LOSS_CACHE; ++k)
+      if (t + k * nt < n)
+      {
+        sf += expf(float(double(fc[k]) - mf));
+        sp += expf(float(uc[k] - mp));
+      }
+    reduce2(sf, sp, false);
+    const double lse = mf + log(sf);
+#pragma unroll
+    for (int k = 0; k < LOSS_CACHE; ++k)
+    {
+      const int i = t + k * nt;
+      if (i < n)
+      {
+        const double p = double(expf(float(uc[k] - mp))) / sp;
+        const double lq = double(fc[k]) - lse;
+        df[i] = float(double(expf(float(lq))) - p);
+        l -= p * lq;
+      }
+    }
+  }
+  else
+  {
+    for (int i = t; i < n; i += nt)
+    {
+      mf = fmax(mf, double(f[i]));
+      mp = fmax(mp, -beta * U[i]);
+    }
+    reduce2(mf, mp, true);
+    for (int i = t; i < n; i += nt)
+    {
+      sf += expf(float(double(f[i]) - mf));
+      sp += expf(float(-beta * U[i] - mp));
+    }
+    reduce2(sf, sp, false);
+    const double lse = mf + log(sf);
+    for (int i = t; i < n; i += nt)
+    {
+      const double p = double(expf(float(-beta * U[i] - mp))) / sp;
+      const double lq = double(f[i]) - lse;
+      df[i] = float(double(expf(float(lq))) - p);
+      l -= p * lq;
+    }
+  }
+  reduce2(l, unused, false);
+  if (t == 0)
+  {
+    loss[0] = l;
+    loss[1] += l;
+  }
+}
+
+// Backward through one hidden layer for the workgroup's rows: dH^T = W dZ^T (W row-major in x out, dZ the LDS tile),
+// dZprev = dH * SiLU'(Zprev) rounded to BF16 into the output tile.
+__device__ __forceinline__ void backLayer(const __bf16* __restrict__ W, const float* __restrict__ Zprev,
+                                          const __bf16 (*din)[LDSW], __bf16 (*dout)[LDSW], int row0, int wv, int lane)
+{
+  f32x4 acc[4][RB];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < RB; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  tileProduct<WIDTH>(W, din, wv, lane, acc);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+  {
+    const int k0 = 64 * wv + 16 * i + 4 * (lane >> 4);
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+    {
+      const int row = 16 * j + (lane & 15);
+      const float4 zv = *reinterpret_cast<const float4*>(Zprev + size_t(row0 + row) * WIDTH + k0);
+      const float z[4] = {zv.x, zv.y, zv.z, zv.w};
+      bf16x4 h;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const float s = sigmoidf(z[r]);
+        h[r] = (__bf16)(acc[i][j][r] * s * (1.0f + z[r] * (1.0f - s)));
+      }
+      *reinterpret_cast<bf16x4*>(&dout[row][k0]) = h;
+    }
+  }
+}
+
+// Backward pass of FROWS rows per workgroup, down to the gradient of the first layer's pre-activations: output layer
+// (dZ3 = df w4 SiLU'(Z3)), then layers 3 and 2 (dZ2, dZ1), all stored feature-major (ld = ldt) for the parameter
+// gradients.
+__global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ df, const float* __restrict__ Z1,
+                                                      const float* __restrict__ Z2, const float* __restrict__ Z3,
+                                                      const __bf16* __restrict__ wb, __bf16* __restrict__ dZ1t,
+                                                      __bf16* __restrict__ dZ2t, __bf16* __restrict__ dZ3t, int ldt)
+{
+  __shared__ __attribute__((aligned(16))) __bf16 bufA[FROWS][LDSW];
+  __shared__ __attribute__((aligned(16))) __bf16 bufB[FROWS][LDSW];
+  __shared__ float dfs[FROWS];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int row0 = blockIdx.x * FROWS;
+  if (t < FROWS) dfs[t] = df[row0 + t];
+  __syncthreads();
+  {
+    // output layer, one thread per hidden unit j
+    const int j = t;
+    const float wj = float(wb[OFF_W4 + j]);
+#pragma unroll 16
+    for (int row = 0; row < FROWS; ++row)
+    {
+      const float z = Z3[size_t(row0 + row) * WIDTH + j], s = sigmoidf(z), d = dfs[row];
+      bufA[row][j] = (__bf16)(d * wj * s * (1.0f + z * (1.0f - s)));
+    }
+  }
+  __syncthreads();
+  storeFeatureMajor(bufA, 64 * wv, 64, dZ3t, ldt, row0, lane);
+  backLayer(wb + OFF_W3, Z2, bufA, bufB, row0, wv, lane);
+  __syncthreads();
+  storeFeatureMajor(bufB, 64 * wv, 64, dZ2t, ldt, row0, lane);
+  backLayer(wb + OFF_W2, Z1, bufB, bufA, row0, wv, lane);
+  __syncthreads();
+  storeFeatureMajor(bufA, 64 * wv, 64, dZ1t, ldt, row0, lane);
+}
+
+// Bias gradients and the output weights' gradient as row sums over the batch, one wavefront per sum (no atomics, a
+// fixed order): db1, db2, db3 = sums of the rows of dZ1t, dZ2t, dZ3t; dw4 = rows of H3t weighted by df; db4 = sum
+// of df.  Every sum is stored, not added: the caller cleared g.
+__global__ __launch_bounds__(256) void k_bias_grads(int n, int ld, const __bf16* __restrict__ dZ1t,
+                                                    const __bf16* __restrict__ dZ2t, const __bf16* __restrict__ dZ3t,
+                                                    const __bf16* __restrict__ H3t, const float* __restrict__ df,
+                                                    float* __restrict__ g)
+{
+  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (wid > 4 * WIDTH) return;
+  const int which = wid / WIDTH, j = wid % WIDTH;
+  // n is a multiple of 16: each lane reads 8 consecutive values per step (16-byte loads, all in flight together)
+  float s = 0.f;
+  if (which == 4)
+  {
+    for (int i = 8 * lane; i < n; i += 512)
+    {
+      const float4 a = *reinterpret_cast<const float4*>(df + i), b = *reinterpret_cast<const float4*>(df + i + 4);
+      s += (a.x + a.y) + (a.z + a.w) + (b.x + b.y) + (b.z + b.w);
+    }
+  }
+  else
+  {
+    const __bf16* row = (which == 0 ? dZ1t : which == 1 ? dZ2t : which == 2 ? dZ3t : H3t) + size_t(j) * ld;
+#pragma unroll 4
+    for (int i = 8 * lane; i < n; i += 512)
+    {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(row + i);
+      if (which == 3)
+      {
+        const float4 a = *reinterpret_cast<const float4*>(df + i), b = *reinterpret_cast<const float4*>(df + i + 4);
+        s += (float(v[0]) * a.x + float(v[1]) * a.y) + (float(v[2]) * a.z + float(v[3]) * a.w) +
+             (float(v[4]) * b.x + float(v[5]) * b.y) + (float(v[6]) * b.z + float(v[7]) * b.w);
+      }
+      else
+        s += (float(v[0]) + float(v[1])) + (float(v[2]) + float(v[3])) + (float(v[4]) + float(v[5])) +
+             (float(v[6]) + float(v[7]));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0)
+  {
+    const size_t off = which == 0 ? OFF_B1 : which == 1 ? OFF_B2 : which == 2 ? OFF_B3 : which == 3 ? OFF_W4 : OFF_B4;
+    g[off + (which == 4 ? 0 : j)] = s;
+  }
+}
+
+__global__ void k_zero(float* p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+// Adam (Kingma & Ba, 2015) on the FP32 masters, then the BF16 copies used by the products (row-major and, for
+// W1..W3, transposed); the gradient is cleared for the next minibatch's atomic sums.
+__global__ void k_adam(int n, float* __restrict__ w, float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, __bf16* __restrict__ wb, __bf16* __restrict__ wt, float lr, float b1,
+                       float b2, float eps, float c1, float c2)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  g[i] = 0.f;
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  wb[i] = (__bf16)wi;
+  const long long ti = transposedIndex(size_t(i));
+  if (ti >= 0) wt[ti] = (__bf16)wi;
+}
+
+__global__ void k_zero_d(double* p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_to_bf16(int n, const float* __restrict__ a, __bf16* __restrict__ b, __bf16* __restrict__ bt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  b[i] = (__bf16)a[i];
+  const long long ti = transposedIndex(size_t(i));
+  if (ti >= 0) bt[ti] = (__bf16)a[i];
+}
+
 }  // namespace
+}  // namespace rg
+
+using namespace rg;
 
 struct rg_mlp
 {
@@ -356,13 +561,16 @@ struct rg_mlp
   int device = 0;
   float* w = nullptr;   // FP32 masters
   __bf16* wb = nullptr; // BF16 copies
+  __bf16* wt = nullptr; // BF16 copies of W1..W3 transposed (out x in)
   float* g = nullptr;   // gradients
   float* m = nullptr;   // Adam moments
   float* v = nullptr;
   long long adamStep = 0;
   // activations for the current batch capacity
   int cap = 0;
-  __bf16 *X = nullptr, *H1 = nullptr, *H2 = nullptr, *H3 = nullptr, *dZa = nullptr, *dZb = nullptr;
+  // layer inputs and pre-activation gradients feature-major (features x cap), for the weight gradients
+  __bf16 *Xt = nullptr, *H1t = nullptr, *H2t = nullptr, *H3t = nullptr, *dZ1t = nullptr, *dZ2t = nullptr,
+         *dZ3t = nullptr;
   float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *f = nullptr, *df = nullptr;
   double *frac = nullptr, *U = nullptr, *loss = nullptr;
 };
@@ -372,10 +580,11 @@ namespace
 
 void freeBatch(rg_mlp* p)
 {
-  void* bufs[] = {p->X, p->H1, p->H2, p->H3, p->dZa, p->dZb, p->Z1, p->Z2, p->Z3, p->f, p->df, p->frac, p->U, p->loss};
+  void* bufs[] = {p->Xt, p->H1t, p->H2t, p->H3t, p->dZ1t, p->dZ2t, p->dZ3t, p->Z1, p->Z2, p->Z3, p->f, p->df,
+                  p->frac, p->U, p->loss};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  p->X = p->H1 = p->H2 = p->H3 = p->dZa = p->dZb = nullptr;
+  p->Xt = p->H1t = p->H2t = p->H3t = p->dZ1t = p->dZ2t = p->dZ3t = nullptr;
   p->Z1 = p->Z2 = p->Z3 = p->f = p->df = nullptr;
   p->frac = p->U = p->loss = nullptr;
   p->cap = 0;
@@ -384,17 +593,18 @@ void freeBatch(rg_mlp* p)
 rg_status ensureBatch(rg_mlp* p, int n)
 {
   rg_ctx* ctx = p->ctx;
-  const int need = (n + 63) / 64 * 64;
+  const int need = (n + FROWS - 1) / FROWS * FROWS;
   if (p->cap >= need) return RG_OK;
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   freeBatch(p);
   const size_t nb = need;
-  RG_HIP(ctx, hipMalloc(&p->X, nb * NIN * sizeof(__bf16)));
-  RG_HIP(ctx, hipMalloc(&p->H1, nb * WIDTH * sizeof(__bf16)));
-  RG_HIP(ctx, hipMalloc(&p->H2, nb * WIDTH * sizeof(__bf16)));
-  RG_HIP(ctx, hipMalloc(&p->H3, nb * WIDTH * sizeof(__bf16)));
-  RG_HIP(ctx, hipMalloc(&p->dZa, nb * WIDTH * sizeof(__bf16)));
-  RG_HIP(ctx, hipMalloc(&p->dZb, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->Xt, nb * NIN * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->H1t, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->H2t, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->H3t, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->dZ1t, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->dZ2t, nb * WIDTH * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->dZ3t, nb * WIDTH * sizeof(__bf16)));
   RG_HIP(ctx, hipMalloc(&p->Z1, nb * WIDTH * sizeof(float)));
   RG_HIP(ctx, hipMalloc(&p->Z2, nb * WIDTH * sizeof(float)));
   RG_HIP(ctx, hipMalloc(&p->Z3, nb * WIDTH * sizeof(float)));
@@ -407,36 +617,35 @@ rg_status ensureBatch(rg_mlp* p, int n)
   return RG_OK;
 }
 
-// forward pass of nrow (multiple of 64) rows whose fractional coordinates are in p->frac
-rg_status forward(rg_mlp* p, int nrow)
+// Forward pass of nrow rows (a multiple of FROWS, at most the batch capacity): fractional coordinates from p->frac,
+// or (perm != nullptr) gathered from fracAll through perm with the energies copied from Uall to p->U.  train keeps
+// what the backward pass needs.
+rg_status forward(rg_mlp* p, int nrow, bool train, const int* perm = nullptr, const double* fracAll = nullptr,
+                  const double* Uall = nullptr)
 {
-  hipStream_t st = p->ctx->stream;
-  const dim3 blk(256);
-  hipLaunchKernelGGL(k_features, dim3((nrow + 3) / 4), dim3(64, 4), 0, st, nrow, p->frac, p->X);
-  const dim3 grid(WIDTH / 64, nrow / 64);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_BIAS_SILU>), grid, blk, 0, st, nrow, WIDTH, NIN, NIN, p->X, NIN,
-                     p->wb + OFF_W1, WIDTH, p->w + OFF_B1, nullptr, p->Z1, p->H1);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_BIAS_SILU>), grid, blk, 0, st, nrow, WIDTH, WIDTH, WIDTH, p->H1, WIDTH,
-                     p->wb + OFF_W2, WIDTH, p->w + OFF_B2, nullptr, p->Z2, p->H2);
-  hipLaunchKernelGGL((k_gemm<false, false, EPI_BIAS_SILU>), grid, blk, 0, st, nrow, WIDTH, WIDTH, WIDTH, p->H2, WIDTH,
-                     p->wb + OFF_W3, WIDTH, p->w + OFF_B3, nullptr, p->Z3, p->H3);
-  hipLaunchKernelGGL(k_out_forward, dim3((nrow + 3) / 4), blk, 0, st, nrow, p->H3, p->wb + OFF_W4, p->w + OFF_B4, p->f);
+  if (nrow % FROWS != 0 || nrow > p->cap) return fail(p->ctx, RG_ERR_INVALID, "mlp forward: %d rows", nrow);
+  const double* fr = perm ? fracAll : p->frac;
+  if (train)
+    hipLaunchKernelGGL(k_mlp_forward<true>, dim3(nrow / FROWS), dim3(256), 0, p->ctx->stream, fr, perm, Uall, p->U,
+                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f);
+  else
+    hipLaunchKernelGGL(k_mlp_forward<false>, dim3(nrow / FROWS), dim3(256), 0, p->ctx->stream, fr, perm, Uall, p->U,
+                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f);
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
 }
 
-// Weight gradient G (M x WIDTH) += A^T dZ over n rows, A stored n x M, split over K chunks of kc rows whose partial
-// products are added atomically.  The chunk is tuned online among 256 .. 4096 rows (results differ only in the
-// order of the FP32 atomic additions, which is unordered anyway).
-rg_status weightGrad(rg_ctx* ctx, int M, int n, const __bf16* A, const __bf16* dZ, float* G)
+// Weight gradient G (M x WIDTH) += A dZ^T over n rows, with A (M x n) and dZ (WIDTH x n) feature-major (row stride
+// ld), split over K chunks of kc rows whose partial products are added atomically.  The chunk is tuned online among
+// 256 .. 4096 rows (results differ only in the order of the FP32 atomic additions, which is unordered anyway).
+rg_status weightGrad(rg_ctx* ctx, int M, int n, int ld, const __bf16* A, const __bf16* dZ, float* G)
 {
   static const int kChunks[5] = {256, 512, 1024, 2048, 4096};
   auto launch = [&](int kc)
   {
     kc = std::min(kc, n);
     const dim3 grid(WIDTH / 64, M / 64, (n + kc - 1) / kc);
-    hipLaunchKernelGGL((k_gemm<true, false, EPI_ATOMIC_F32>), grid, dim3(256), 0, ctx->stream, M, WIDTH, n, kc, A, M,
-                       dZ, WIDTH, nullptr, nullptr, G, nullptr);
+    hipLaunchKernelGGL(k_weight_grad, grid, dim3(256), 0, ctx->stream, M, WIDTH, n, kc, A, ld, dZ, ld, G);
     return hipGetLastError();
   };
   Autotuner& tuner = Autotuner::instance();
@@ -453,26 +662,20 @@ rg_status weightGrad(rg_ctx* ctx, int M, int n, const __bf16* A, const __bf16* d
   return RG_OK;
 }
 
-// loss and gradients of a batch of n rows (n multiple of 64) already in p->frac / p->U
+// Loss and gradients (added into p->g, which the caller has cleared) of a batch of n rows (a multiple of FROWS)
+// whose forward pass ran with train = true: the loss kernel, the fused backward pass and the three weight gradients.
 rg_status backward(rg_mlp* p, int n, double beta)
 {
   hipStream_t st = p->ctx->stream;
-  const dim3 blk(256);
-  hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), blk, 0, st, p->g, int(NPARAM));
-  hipLaunchKernelGGL(k_loss, dim3(1), dim3(1024), 0, st, n, p->f, p->U, beta, p->df, p->loss);
-  hipLaunchKernelGGL(k_out_backward, dim3((n + OB_ROWS - 1) / OB_ROWS), dim3(WIDTH), 0, st, n, p->H3, p->Z3,
-                     p->wb + OFF_W4, p->df, p->g + OFF_W4, p->g + OFF_B4, p->dZa, p->g + OFF_B3);
-  // layer 3: dW3 = H2^T dZ3; dZ2 = (dZ3 W3^T) * SiLU'(Z2) with db2 = column sums of dZ2
+  hipLaunchKernelGGL(k_loss, dim3(1), dim3(LOSS_THREADS), 0, st, n, p->f, p->U, beta, p->df, p->loss);
+  hipLaunchKernelGGL(k_mlp_backward, dim3(n / FROWS), dim3(256), 0, st, p->df, p->Z1, p->Z2, p->Z3, p->wb, p->dZ1t,
+                     p->dZ2t, p->dZ3t, p->cap);
+  hipLaunchKernelGGL(k_bias_grads, dim3((4 * WIDTH + 1 + 3) / 4), dim3(256), 0, st, n, p->cap, p->dZ1t, p->dZ2t,
+                     p->dZ3t, p->H3t, p->df, p->g);
   rg_status ws;
-  if ((ws = weightGrad(p->ctx, WIDTH, n, p->H2, p->dZa, p->g + OFF_W3)) != RG_OK) return ws;
-  hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
-                     p->dZa, WIDTH, p->wb + OFF_W3, WIDTH, nullptr, p->Z2, p->g + OFF_B2, p->dZb);
-  // layer 2: dW2 = H1^T dZ2; dZ1 = (dZ2 W2^T) * SiLU'(Z1) with db1
-  if ((ws = weightGrad(p->ctx, WIDTH, n, p->H1, p->dZb, p->g + OFF_W2)) != RG_OK) return ws;
-  hipLaunchKernelGGL((k_gemm<false, true, EPI_DSILU>), dim3(WIDTH / 64, n / 64), blk, 0, st, n, WIDTH, WIDTH, WIDTH,
-                     p->dZb, WIDTH, p->wb + OFF_W2, WIDTH, nullptr, p->Z1, p->g + OFF_B1, p->dZa);
-  // layer 1: dW1 = X^T dZ1
-  if ((ws = weightGrad(p->ctx, NIN, n, p->X, p->dZa, p->g + OFF_W1)) != RG_OK) return ws;
+  if ((ws = weightGrad(p->ctx, WIDTH, n, p->cap, p->H2t, p->dZ3t, p->g + OFF_W3)) != RG_OK) return ws;
+  if ((ws = weightGrad(p->ctx, WIDTH, n, p->cap, p->H1t, p->dZ2t, p->g + OFF_W2)) != RG_OK) return ws;
+  if ((ws = weightGrad(p->ctx, NIN, n, p->cap, p->Xt, p->dZ1t, p->g + OFF_W1)) != RG_OK) return ws;
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
 }
@@ -493,6 +696,7 @@ rg_status rg_mlp_create(rg_ctx* ctx, unsigned long long seed, rg_mlp** out)
   p->device = ctx->device;
   RG_HIP(ctx, hipMalloc(&p->w, NPARAM * sizeof(float)));
   RG_HIP(ctx, hipMalloc(&p->wb, NPARAM * sizeof(__bf16)));
+  RG_HIP(ctx, hipMalloc(&p->wt, NTRANS * sizeof(__bf16)));
   RG_HIP(ctx, hipMalloc(&p->g, NPARAM * sizeof(float)));
   RG_HIP(ctx, hipMalloc(&p->m, NPARAM * sizeof(float)));
   RG_HIP(ctx, hipMalloc(&p->v, NPARAM * sizeof(float)));
@@ -518,7 +722,9 @@ rg_status rg_mlp_create(rg_ctx* ctx, unsigned long long seed, rg_mlp** out)
   fill(OFF_W3, size_t(WIDTH) * WIDTH, WIDTH);
   fill(OFF_W4, WIDTH, WIDTH);
   RG_HIP(ctx, hipMemcpyAsync(p->w, w.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->wb);
+  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->wb,
+                     p->wt);
+  hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->g, int(NPARAM));
   hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->m, int(NPARAM));
   hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->v, int(NPARAM));
   RG_HIP(ctx, hipGetLastError());
@@ -534,7 +740,7 @@ rg_status rg_mlp_destroy(rg_mlp* p)
   // the context (and its stream) may already be gone: wait for the whole device instead
   st = rgReleaseStatus(hipDeviceSynchronize(), st);
   freeBatch(p);
-  void* bufs[] = {p->w, p->wb, p->g, p->m, p->v};
+  void* bufs[] = {p->w, p->wb, p->wt, p->g, p->m, p->v};
   for (void* b : bufs)
     if (b) st = rgReleaseStatus(hipFree(b), st);
   delete p;
@@ -553,7 +759,8 @@ rg_status rg_mlp_set_params(rg_ctx* ctx, rg_mlp* p, const float* in)
 {
   if (!ctx || !p || !in) return fail(ctx, RG_ERR_INVALID, "rg_mlp_set_params: invalid arguments");
   RG_HIP(ctx, hipMemcpyAsync(p->w, in, NPARAM * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->wb);
+  hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->wb,
+                     p->wt);
   RG_HIP(ctx, hipGetLastError());
   RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return RG_OK;
@@ -572,11 +779,11 @@ rg_status rg_mlp_forward(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, doub
   for (int i0 = 0; i0 < n; i0 += chunk)
   {
     const int c = std::min(chunk, n - i0);
-    const int rows = (c + 63) / 64 * 64;
+    const int rows = (c + FROWS - 1) / FROWS * FROWS;
     RG_HIP(ctx, hipMemsetAsync(p->frac, 0, size_t(rows) * 3 * sizeof(double), ctx->stream));
     RG_HIP(ctx, hipMemcpyAsync(p->frac, frac + 3 * size_t(i0), size_t(c) * 3 * sizeof(double), hipMemcpyHostToDevice,
                                ctx->stream));
-    if ((st = forward(p, rows)) != RG_OK) return st;
+    if ((st = forward(p, rows, false)) != RG_OK) return st;
     RG_HIP(ctx, hipMemcpyAsync(f.data(), p->f, c * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     RG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < c; ++i) logdensity[i0 + i] = f[i];
@@ -594,7 +801,8 @@ rg_status rg_mlp_gradients(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, co
   if (st != RG_OK) return st;
   RG_HIP(ctx, hipMemcpyAsync(p->frac, frac, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(p->U, energy, size_t(n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  if ((st = forward(p, n)) != RG_OK) return st;
+  hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->g, int(NPARAM));
+  if ((st = forward(p, n, true)) != RG_OK) return st;
   if ((st = backward(p, n, beta)) != RG_OK) return st;
   RG_HIP(ctx, hipMemcpyAsync(grads, p->g, NPARAM * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -632,6 +840,8 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
   };
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const int nb = n / batch;
+  // the gradient sums start from zero; k_adam clears them again after every step
+  hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->g, int(NPARAM));
   for (int ep = 0; ep < epochs && e == hipSuccess && st == RG_OK; ++ep)
   {
     for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[rnd() % (unsigned long long)(i + 1)]);
@@ -641,13 +851,13 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, ctx->stream, p->loss, 2);
     for (int b = 0; b < nb; ++b)
     {
-      hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, ctx->stream, batch,
-                         dperm + size_t(b) * batch, dfrac, dU, p->frac, p->U);
-      if ((st = forward(p, batch)) != RG_OK || (st = backward(p, batch, beta)) != RG_OK) break;
+      if ((st = forward(p, batch, true, dperm + size_t(b) * batch, dfrac, dU)) != RG_OK ||
+          (st = backward(p, batch, beta)) != RG_OK)
+        break;
       p->adamStep += 1;
       const float c1 = 1.f - std::pow(b1, float(p->adamStep)), c2 = 1.f - std::pow(b2, float(p->adamStep));
       hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->g, p->m,
-                         p->v, p->wb, float(lr), b1, b2, eps, c1, c2);
+                         p->v, p->wb, p->wt, float(lr), b1, b2, eps, c1, c2);
     }
     e = hipGetLastError();
     double l[2] = {0.0, 0.0};
