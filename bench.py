@@ -957,7 +957,7 @@ def main():
                     help="measure the headline grid build and the triquintic grids of the same workload only")
     ap.add_argument("--widom-insertions", type=int, default=1 << 20)
     ap.add_argument("--gcmc-systems", type=int, default=256)
-    ap.add_argument("--gcmc-equilibration", type=int, default=600)
+    ap.add_argument("--gcmc-equilibration", type=int, default=1000)
     ap.add_argument("--gcmc-production", type=int, default=400)
     ap.add_argument("--tau", type=float, default=0.5)
     ap.add_argument("--kfirst-sweep", default="10,4,2,1",

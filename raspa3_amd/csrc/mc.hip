@@ -47,8 +47,7 @@ __device__ void siteEnergyWave(const SysDev& S, double px, double py, double pz,
   }
   else
   {
-    for (int a = lane; a < S.nfw; a += 64)
-    {
+    forNearFramework(S, px, py, pz, lane, [&](int a) {
       double dx = S.fwx[a] - px, dy = S.fwy[a] - py, dz = S.fwz[a] - pz;
       rgd::pbc(S.box, dx, dy, dz);
       const double rr = dx * dx + dy * dy + dz * dz;
@@ -63,7 +62,7 @@ __device__ void siteEnergyWave(const SysDev& S, double px, double py, double pz,
         const double r = sqrt(rr);
         ecf += rgd::coul0(S.alpha, 1.0, r, S.coulomb * S.fwq[a] * q);
       }
-    }
+    });
   }
   for (int a = lane; a < S.nads; a += 64)
   {

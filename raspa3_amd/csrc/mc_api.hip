@@ -118,28 +118,75 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
     dp = tmp;
     h.pairs = dp;
   }
-  // framework atoms (direct path)
+  // framework atoms (direct path), sorted by cell when the box holds at least three cells of one cutoff width per
+  // axis (perpendicular widths); the sums then visit only the 27 cells around a site
   if (fw)
   {
     const int n = fw->n;
-    std::vector<double> x(n), y(n), z(n), q(n);
-    double net = 0.0;
-    for (int a = 0; a < n; ++a)
-    {
-      x[a] = fw->pos[3 * a];
-      y[a] = fw->pos[3 * a + 1];
-      z[a] = fw->pos[3 * a + 2];
-      q[a] = fw->charge[a];
-      net += q[a];
-    }
     for (int t : fw->type)
       if (t < 0 || t >= ff->n) return bail(fail(ctx, RG_ERR_INVALID, "rg_system_create: framework type out of range"));
+    const double rc = std::max(ff->rc_framework, ff->use_charge ? ff->rc_coulomb : 0.0);
+    int nc[3];
+    bool cells = rc > 0.0;
+    for (int ax = 0; ax < 3; ++ax)
+    {
+      const double* v = h.box.inv;
+      const double width = 1.0 / std::sqrt(v[ax] * v[ax] + v[3 + ax] * v[3 + ax] + v[6 + ax] * v[6 + ax]);
+      nc[ax] = int(std::min(1024.0, std::floor(width / std::max(rc, 1e-12))));
+      cells = cells && nc[ax] >= 3;
+    }
+    std::vector<int> cellOf(n, 0), start;
+    if (cells)
+    {
+      for (int a = 0; a < n; ++a)
+      {
+        int c[3];
+        for (int ax = 0; ax < 3; ++ax)
+        {
+          double sv = h.box.inv[ax] * fw->pos[3 * a] + h.box.inv[3 + ax] * fw->pos[3 * a + 1] +
+                      h.box.inv[6 + ax] * fw->pos[3 * a + 2];
+          sv -= std::floor(sv);
+          c[ax] = std::min(nc[ax] - 1, int(sv * nc[ax]));
+        }
+        cellOf[a] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+      }
+      start.assign(size_t(nc[0]) * nc[1] * nc[2] + 1, 0);
+      for (int a = 0; a < n; ++a) start[cellOf[a] + 1] += 1;
+      for (size_t c = 1; c < start.size(); ++c) start[c] += start[c - 1];
+    }
+    // atom order: by cell (stable), or as given
+    std::vector<int> order(n);
+    {
+      std::vector<int> fill(start.begin(), start.end());
+      for (int a = 0; a < n; ++a) order[cells ? fill[cellOf[a]]++ : a] = a;
+    }
+    std::vector<double> x(n), y(n), z(n), q(n);
+    std::vector<int> type(n);
+    double net = 0.0;
+    for (int i = 0; i < n; ++i)
+    {
+      const int a = order[i];
+      x[i] = fw->pos[3 * a];
+      y[i] = fw->pos[3 * a + 1];
+      z[i] = fw->pos[3 * a + 2];
+      q[i] = fw->charge[a];
+      type[i] = fw->type[a];
+      net += fw->charge[a];
+    }
     double *dx, *dy, *dz, *dq;
     int* dt;
     if ((st = devUpload(s, dx, x.data(), n)) != RG_OK || (st = devUpload(s, dy, y.data(), n)) != RG_OK ||
         (st = devUpload(s, dz, z.data(), n)) != RG_OK || (st = devUpload(s, dq, q.data(), n)) != RG_OK ||
-        (st = devUpload(s, dt, fw->type.data(), n)) != RG_OK)
+        (st = devUpload(s, dt, type.data(), n)) != RG_OK)
       return bail(st);
+    h.cellStart = nullptr;
+    if (cells)
+    {
+      int* dstart;
+      if ((st = devUpload(s, dstart, start.data(), start.size())) != RG_OK) return bail(st);
+      h.cellStart = dstart;
+      for (int ax = 0; ax < 3; ++ax) h.ncell[ax] = nc[ax];
+    }
     h.nfw = n;
     h.fwx = dx;
     h.fwy = dy;

@@ -117,8 +117,7 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
   }
   if (gi < 0)
   {
-    for (int a = lane; a < S.nfw; a += 64)
-    {
+    forNearFramework(S, px, py, pz, lane, [&](int a) {
       double dx = S.fwx[a] - px, dy = S.fwy[a] - py, dz = S.fwz[a] - pz;
       rgd::pbc(S.box, dx, dy, dz);
       const double rr = dx * dx + dy * dy + dz * dz;
@@ -129,7 +128,7 @@ __device__ double siteExternal(const SysDev& S, int nads, double px, double py, 
         e += v;
       }
       if (S.useCharge && rr < S.rcc2) e += rgd::coul0(S.alpha, 1.0, sqrt(rr), S.coulomb * S.fwq[a] * q);
-    }
+    });
   }
   // adsorbate atoms, four per lane and pass so that one memory latency covers their loads
   constexpr int U = 4;

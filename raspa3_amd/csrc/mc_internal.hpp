@@ -19,10 +19,14 @@ struct SysDev
   int useCharge;
   int ntypes;
   const rgd::Pair* pairs;  // ntypes * ntypes
-  // framework atoms for the direct sum (SoA)
+  // framework atoms for the direct sum (SoA), sorted by cell when cellStart is set: cell (c0, c1, c2) of ncell
+  // cells per axis (fractional cells, each at least the cutoff wide) holds atoms cellStart[c] .. cellStart[c + 1] - 1,
+  // c = (c2 ncell[1] + c1) ncell[0] + c0
   int nfw;
   const double *fwx, *fwy, *fwz, *fwq;
   const int* fwt;
+  int ncell[3];
+  const int* cellStart;
   // interpolation grids: gridOfType[type] indexes grids (or -1); coulGrid indexes the unit-charge grid (or -1)
   const int* gridOfType;
   const rgd::GridView* grids;
@@ -42,6 +46,43 @@ struct SysDev
   double singleIon;    // sum_k f_k (a unit charge)
   double netCharge;    // framework + adsorbates
 };
+
+// Calls f(a) for the framework atoms that can lie within the cutoff of (px, py, pz), lanes striding over them:
+// with a cell list the atoms of the 27 cells around the point's cell (every axis has at least three cells, each at
+// least one cutoff wide, so the 27 are distinct and hold every atom within the cutoff under the minimum image),
+// otherwise all atoms.  The caller applies the cutoff to the minimum-image distance, as for a full scan.
+template <class F>
+__device__ __forceinline__ void forNearFramework(const SysDev& S, double px, double py, double pz, int lane, F&& f)
+{
+  if (!S.cellStart)
+  {
+    for (int a = lane; a < S.nfw; a += 64) f(a);
+    return;
+  }
+  int c[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+  {
+    double s = S.box.inv[ax] * px + S.box.inv[3 + ax] * py + S.box.inv[6 + ax] * pz;
+    s -= floor(s);
+    c[ax] = min(S.ncell[ax] - 1, int(s * S.ncell[ax]));
+  }
+  for (int d2 = -1; d2 <= 1; ++d2)
+  {
+    const int c2 = (c[2] + d2 + S.ncell[2]) % S.ncell[2];
+    for (int d1 = -1; d1 <= 1; ++d1)
+    {
+      const int c1 = (c[1] + d1 + S.ncell[1]) % S.ncell[1];
+      for (int d0 = -1; d0 <= 1; ++d0)
+      {
+        const int c0 = (c[0] + d0 + S.ncell[0]) % S.ncell[0];
+        const int cell = (c2 * S.ncell[1] + c1) * S.ncell[0] + c0;
+        const int end = S.cellStart[cell + 1];
+        for (int a = S.cellStart[cell] + lane; a < end; a += 64) f(a);
+      }
+    }
+  }
+}
 
 }  // namespace rg
 

@@ -143,6 +143,40 @@ def test_trial_energies_match_oracle_config1(gpu_ctx):
     assert ov[:5].all()
 
 
+def test_trial_energies_with_cell_list_match_oracle(gpu_ctx):
+    """Direct framework sums through the framework cell list: charged ITQ-29 5 x 5 x 5 (9000 sites, 59.4 A) has
+    four cells of at least one cutoff per axis, so each site visits the 27 cells around it; random sites and sites
+    0.4 A from framework O atoms (overlaps) against the oracle's scan over every atom (minimum image).  Configuration 1
+    (30 A along c: two cells) keeps the full scan, covered by the test above."""
+    print("case: trial energies, framework cell list (ITQ-29 5x5x5)", flush=True)
+    spec = w.forcefield_zeolite()
+    fw, _ = w.framework_itq29(spec, 5)
+    ff = ra.ForceField(gpu_ctx, spec)
+    s = ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=ra.Framework(gpu_ctx, fw), kmax=(0, 0, 0), capacity=64)
+    pairs = oracle.pairs_from_spec(spec)
+    rng = np.random.default_rng(12)
+    ns = 400
+    sp = rng.uniform(-0.2, 1.2, (ns, 3)) @ fw.cell.T  # also outside the cell: the site's cell is taken modulo 1
+    oxygens = np.flatnonzero(fw.type == spec.index("O"))
+    sp[:8] = fw.pos[rng.choice(oxygens, 8, replace=False)] + 0.4 / np.sqrt(3.0)
+    st = rng.choice([spec.index("C_co2"), spec.index("O_co2")], ns).astype(np.int32)
+    sq = np.where(st == spec.index("C_co2"), 0.6512, -0.3256)
+    e, ov = s.trial_energies(sp, st, sq, np.full(ns, 1000, np.int32))
+    checked = 0
+    for i in range(ns):
+        site = dict(pos=sp[i:i + 1], type=st[i:i + 1], charge=sq[i:i + 1], molecule=np.array([1000], np.int32))
+        fo = oracle.framework_energy(fw.cell, fw.rect, fw.pos, fw.type, fw.charge, pairs, 12.0, 12.0, spec.alpha,
+                                     True, site)
+        if fo is None:
+            assert ov[i], i
+            continue
+        assert not ov[i], i
+        ref = np.array([fo[0], fo[1]])
+        assert np.all(np.abs(e[i, :2] - ref) <= 1e-9 * np.maximum(1.0, np.abs(ref))), (i, e[i], ref)
+        checked += 1
+    assert ov[:8].all() and checked > 150  # about 40 % of random points in the zeolite overlap
+
+
 def test_ewald_delta_accept_matches_oracle(gpu_ctx):
     """Insertion, displacement and deletion of CO2 in the charged framework at kmax = 8 (2600 k-vectors):
     rg_ewald_delta equals the oracle's energyDifferenceEwaldFourier, and after rg_ewald_accept the stored
