@@ -1275,7 +1275,9 @@ __device__ __forceinline__ int kthMoleculeOfComponent(const McSys& Y, int nmol, 
 
 // Inlined into k_gcmc: as an outlined function (the compiler's choice once the step grew) it reached the kernel's LDS
 // descriptors and scratch through generic pointers, one flat access per field.
-template <bool CHAIN>
+// REINS: the kernel instance carries the reinsertion move (runs with p_reinsertion > 0; without it the move's code,
+// and the registers it holds, are left out)
+template <bool CHAIN, bool REINS>
 __device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const SysDev& S, const McSys& Y, int sys, Live& L, int lane, WaveScratch& W,
                          const Team& T)
 {
@@ -1412,7 +1414,7 @@ __device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const Sy
   // hipcc (ROCm 7.2) fails on k_gcmc<true> with "Illegal instruction detected: Operand has incorrect register class"
   // (a V_CMP on src_shared_base), so the host refuses p_reinsertion > 0 for them.
   const double pRe = pRot + R.pReinsertion;
-  const bool reins = !CHAIN && uMove < pRe;
+  const bool reins = REINS && !CHAIN && uMove < pRe;
   const bool insert = !reins && uMove < pRe + 0.5 * (1.0 - pRe);
   L.trials[reins ? 4 : (insert ? 0 : 1)] += 1;
   const int nc = L.ncomp[ci];
@@ -1589,7 +1591,7 @@ __global__ __launch_bounds__(256) void k_mc_budget(const McRun* __restrict__ Rp,
 // their summed cost stays within the budget k_mc_budget wrote, so lightly loaded systems do more cycles in the time
 // the most loaded one takes for its first; the samples of the launch go to the block of its sampled index.
 // Otherwise every system runs 'cycles' cycles and a sample goes to the block of the system's sampled-cycle count.
-template <bool CHAIN, bool BAL>
+template <bool CHAIN, bool BAL, bool REINS>
 __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysDev* __restrict__ Sarr,
                                               McSys* __restrict__ Yarr, int nsys, int cycles, int sample, int nw,
                                               const McCtl* __restrict__ ctl)
@@ -1649,7 +1651,7 @@ __global__ __launch_bounds__(256) void k_gcmc(const McRun* __restrict__ Rp, SysD
     if (BAL) spent += balancedCycleCost(R, L.nmol);
     for (int s = 0; s < nsteps; ++s)
     {
-      gcmcStep<CHAIN>(R, S, Y, sys, L, lane, scratch[w], T);
+      gcmcStep<CHAIN, REINS>(R, S, Y, sys, L, lane, scratch[w], T);
       // the team's first wavefront wrote the adsorbate arrays; the next step reads them on every wavefront
       if (nw > 1)
         __syncthreads();
@@ -1779,8 +1781,10 @@ template __global__ void k_widom<false>(const McRun*, const SysDev*, const McSys
                                         double*, double*);
 template __global__ void k_widom<true>(const McRun*, const SysDev*, const McSys*, int, long long, int, double*,
                                        double*, double*);
-template __global__ void k_gcmc<false, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
-template __global__ void k_gcmc<false, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
-template __global__ void k_gcmc<true, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<false, false, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<false, false, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<false, true, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<false, true, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<true, false, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
 
 }  // namespace rg

@@ -10,7 +10,7 @@
 
 namespace rg
 {
-template <bool CHAIN, bool BAL>
+template <bool CHAIN, bool BAL, bool REINS>
 __global__ void k_gcmc(const McRun* Rp, SysDev* Sarr, McSys* Yarr, int nsys, int cycles, int sample, int nw,
                        const McCtl* ctl);
 __global__ void k_mc_budget(const McRun* Rp, const McSys* Yarr, int nsys, McCtl* ctl, int sample);
@@ -398,7 +398,11 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   {
     const int nw = kNw[cand], wpb = kWpb[cand];
     if (bal) hipLaunchKernelGGL(k_mc_budget, dim3(1), dim3(256), 0, ctx->stream, mc->dRun, mc->dY, nsys, mc->dCtl, sample);
-    auto kernel = chain ? k_gcmc<true, false> : bal ? k_gcmc<false, true> : k_gcmc<false, false>;
+    // kernel instances: chains (no balancing, no reinsertion) or rigid molecules with or without either
+    const bool reins = mc->run.pReinsertion > 0.0;
+    auto kernel = chain  ? k_gcmc<true, false, false>
+                  : bal  ? (reins ? k_gcmc<false, true, true> : k_gcmc<false, true, false>)
+                         : (reins ? k_gcmc<false, false, true> : k_gcmc<false, false, false>);
     hipLaunchKernelGGL(kernel, dim3((nsys + wpb - 1) / wpb), dim3(64 * nw * wpb), nw * wpb * sizeof(WaveScratch),
                        ctx->stream, mc->dRun, mc->dS, mc->dY, nsys, 1, sample, nw, static_cast<const McCtl*>(mc->dCtl));
     return hipGetLastError();
