@@ -352,8 +352,23 @@ def bench_grids_quintic(ctx, g1, args):
     return out
 
 
+def widom_mu(lw, ew, temperature):
+    """The reference's Widom statistics (rg_widom_statistics: <W> and the excess chemical potential with their 95 %
+    intervals over 5 blocks) of per-insertion log Rosenbluth weights lw and Ewald Fourier differences ew (rigid
+    molecules, W_IG = 1); energies in K."""
+    import raspa3_amd as ra
+
+    wts = np.exp(np.asarray(lw) - ra.beta(temperature) * np.asarray(ew))
+    nb = len(wts) // 5
+    blocks = wts[:5 * nb].reshape(5, nb)
+    st = ra.widom_statistics(np.full(5, float(nb)), blocks.sum(1), np.zeros(5), temperature)
+    return dict(rosenbluth_weight=st["rosenbluth_weight"], rosenbluth_weight_ci=st["rosenbluth_weight_ci"],
+                mu_excess_K=st["mu_excess"] * ra.ENERGY_TO_KELVIN, mu_excess_ci_K=st["mu_excess_ci"] * ra.ENERGY_TO_KELVIN)
+
+
 def bench_trial_energies(ctx, g1, args):
-    """Widom insertions through the device kernel: trial site energies per second."""
+    """Widom insertions through the device kernel: trial site energies per second, and the reference's Widom
+    statistics of the insertions (Rosenbluth weight, excess chemical potential)."""
     import torch
 
     import raspa3_amd as ra
@@ -387,12 +402,13 @@ def bench_trial_energies(ctx, g1, args):
     ctx.sync()
     dt = ev0.elapsed_time(ev1) / 1000.0
     lw = outs[0].cpu().numpy()
+    stats0 = widom_mu(lw, outs[2].cpu().numpy(), 300.0)
     t = time.perf_counter()
     ra.widom(ctx, s0, comp0, p0, 4096, n)
     dt_host = time.perf_counter() - t
     sites = n * p0["k_first"]
     out["ch4_config0"] = dict(insertions=n, seconds=round(dt, 5), sites_per_s=sites / dt,
-                              sites_per_s_host_buffers=sites / dt_host, widom_weight=float(np.exp(lw).mean()),
+                              sites_per_s_host_buffers=sites / dt_host, widom=stats0,
                               roofline=dict(bound="hbm", achieved=sites / dt * GRID_BYTES_PER_SITE / 1e9,
                                             peak=HBM_PEAK_GBS, unit="GB/s",
                                             frac=sites / dt * GRID_BYTES_PER_SITE / 1e9 / HBM_PEAK_GBS))
@@ -414,8 +430,9 @@ def bench_trial_energies(ctx, g1, args):
     ctx.sync()
     dt = ev0.elapsed_time(ev1) / 1000.0
     sites = n1 * (p1["k_first"] + 2 * p1["k_orient"])
+    stats1 = widom_mu(outs[0][:n1].cpu().numpy(), outs[2][:n1].cpu().numpy(), 300.0)
     out["co2_config1"] = dict(insertions=n1, seconds=round(dt, 5), sites_per_s=sites / dt,
-                              ewald_deltas_per_s=n1 / dt, n_kvectors=s1.info()["nk"])
+                              ewald_deltas_per_s=n1 / dt, n_kvectors=s1.info()["nk"], widom=stats1)
     log(f"trial energies CO2 config 1: {sites / dt:.4e} sites/s, {n1 / dt:.4e} Ewald deltas/s")
     out["unit"] = "trial site energies/s"
     return out
