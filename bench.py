@@ -502,7 +502,8 @@ def bench_gcmc(ctx, args, framework="itq29", tau=None, pressure=1e5):
     from raspa3_amd._lib import RG_GRID_EWALD_REAL, RG_GRID_LENNARD_JONES
 
     T, P = 300.0, pressure
-    spec = w.forcefield_zeolite()
+    n2 = args.gcmc_component == "n2"
+    spec = w.forcefield_zeolite([("N_n2", 36.0, 3.31, -0.482), ("N_com", 0.0, 1.0, 0.964)] if n2 else ())
     tau = args.tau if tau is None else tau
     if framework == "mfi":
         fw, unit = w.framework_mfi(spec, (2, 2, 2))
@@ -517,14 +518,14 @@ def bench_gcmc(ctx, args, framework="itq29", tau=None, pressure=1e5):
     ff = ra.ForceField(ctx, spec)
     fu = ra.Framework(ctx, w.Framework(framework, unit, True, fw.pos[:nuc], fw.type[:nuc], fw.charge[:nuc]))
     dims = ra.grid_dims(unit, 0.2)
-    types = [spec.index("C_co2"), spec.index("O_co2")]
+    types = [spec.index(n) for n in (("N_n2", "N_com") if n2 else ("C_co2", "O_co2"))]
     grids = {t: ra.Grid(ctx, unit, True, dims) for t in types}
     coul = ra.Grid(ctx, unit, True, dims)
     ra.build_grids(ctx, ff, fu, [(g, RG_GRID_LENNARD_JONES, t, 12.0) for t, g in grids.items()] +
                    [(coul, RG_GRID_EWALD_REAL, 0, 12.0)])
     fb = ra.Framework(ctx, fw)
     _, kmax = ra.ewald_parameters(1e-6, 12.0, np.diag(fw.cell))
-    comp = w.co2_reference(spec)
+    comp = w.n2_component(spec) if n2 else w.co2_reference(spec)
     nsys = args.gcmc_systems
     stream = torch.cuda.ExternalStream(ctx.stream, device=torch.device("cuda", 0))
 
@@ -608,7 +609,8 @@ def bench_gcmc(ctx, args, framework="itq29", tau=None, pressure=1e5):
     best = max(matched or learned.values(), key=lambda r: r["accepted_insertions_per_s"])
     best_plain = max([plain] + [r for r in plain_k.values() if is_matched(r)],
                      key=lambda r: r["accepted_insertions_per_s"])
-    return dict(workload=f"CO2 (reference rigid model) in {label}, {T:g} K, {P:g} Pa, "
+    return dict(workload=f"{'N2 (TraPPE, grown from an N site)' if n2 else 'CO2 (reference rigid model)'} in {label}, "
+                         f"{T:g} K, {P:g} Pa, "
                          f"{nsys} systems, {args.gcmc_equilibration} + {args.gcmc_production} cycles, grids at 0.2 A "
                          f"on the unit cell, Ewald kmax {list(kmax)}, 10 orientation trials; plain CBMC with 10 "
                          f"first-bead trials (and with each other count of {args.kfirst_sweep}), the learned "
@@ -752,13 +754,19 @@ def bench_gcmc_mixture(ctx, args):
             for k, sh in enumerate(shape_of):
                 for ci in range(len(comps)):
                     mc.set_proposal(k, *tables[ci][sh], component=ci)
-        mc.run(args.mixture_equilibration, sample=False, graph=True)
-        log(f"gcmc mixture {label}: {args.mixture_equilibration} equilibration cycles done")
+        # launches in chunks of at most 1000, with a progress line after each
+        chunk = 1000
+        for done in range(0, args.mixture_equilibration, chunk):
+            mc.run(min(chunk, args.mixture_equilibration - done), sample=False, graph=True)
+            log(f"gcmc mixture {label}: {min(done + chunk, args.mixture_equilibration)} equilibration launches done")
         st0 = [mc.state(k) for k in range(nsys)]
         base = [(st0[k]["accepted"][0], st0[k]["trials"][0]) for k in range(nsys)]
         ctx.sync()
         t = time.perf_counter()
-        mc.run(args.mixture_production, sample=True, graph=True)
+        for done in range(0, args.mixture_production, chunk):
+            mc.run(min(chunk, args.mixture_production - done), sample=True, graph=True)
+            if args.mixture_production > chunk:
+                log(f"gcmc mixture {label}: {min(done + chunk, args.mixture_production)} production launches done")
         ctx.sync()
         dt = time.perf_counter() - t
         r = gcmc_stats(mc, nsys, base, T, 2, groups, list(cells))
@@ -787,6 +795,13 @@ def bench_gcmc_mixture(ctx, args):
             f"{r['qst_kJmol'][1]:.3f} +- {r['qst_sigma'][1]:.3f} kJ/mol")
         return r
 
+    if args.mixture_method != "both":
+        # one method alone (long runs split over separate invocations); no comparison in this line
+        r = run(args.mixture_method, 2001 if args.mixture_method == "plain" else 2002,
+                args.mixture_method == "learned", 10 if args.mixture_method == "plain" else args.mixture_kfirst)
+        return {args.mixture_method: r, "workload": f"CO2/N2 80:20 at {T:g} K and {P:g} Pa, {nsys} systems in charged "
+                f"ITQ-29 supercells {shapes}, {args.mixture_equilibration} + {args.mixture_production} launches, "
+                f"balanced={not args.no_mixture_balance}", "unit": "accepted insertions/s"}
     plain = run("plain", 2001, False)
     learned = run("learned", 2002, True, args.mixture_kfirst)
     extra = {}
@@ -974,6 +989,8 @@ def main():
                     help="measure the headline grid build and the triquintic grids of the same workload only")
     ap.add_argument("--widom-insertions", type=int, default=1 << 20)
     ap.add_argument("--gcmc-systems", type=int, default=256)
+    ap.add_argument("--gcmc-component", choices=("co2", "n2"), default="co2",
+                    help="adsorbate of the --gcmc-only comparisons (the default legs use CO2)")
     ap.add_argument("--gcmc-equilibration", type=int, default=1000)
     ap.add_argument("--gcmc-production", type=int, default=400)
     ap.add_argument("--tau", type=float, default=0.5)
@@ -996,6 +1013,8 @@ def main():
     ap.add_argument("--mixture-pressure", type=float, default=1e5)
     ap.add_argument("--mixture-voxels", type=int, default=16, help="proposal voxels per unit-cell axis")
     ap.add_argument("--mixture-kfirst", type=int, default=2, help="first-bead trials of the learned mixture run")
+    ap.add_argument("--mixture-method", choices=("both", "plain", "learned"), default="both",
+                    help="run both methods of the mixture and compare them (default), or one of them alone")
     ap.add_argument("--no-mixture-balance", action="store_true",
                     help="one cycle per system and launch in the mixture (no balanced launches)")
     ap.add_argument("--mixture-balance-overhead", type=float, default=256.0,
