@@ -132,7 +132,8 @@ rg_status rg_system_create(rg_ctx* ctx, const rg_forcefield* ff, const double* c
     {
       const double* v = h.box.inv;
       const double width = 1.0 / std::sqrt(v[ax] * v[ax] + v[3 + ax] * v[3 + ax] + v[6 + ax] * v[6 + ax]);
-      nc[ax] = int(std::min(1024.0, std::floor(width / std::max(rc, 1e-12))));
+      // at most 128 cells per axis (2M cells); cells wider than the cutoff only visit more atoms, never fewer
+      nc[ax] = int(std::min(128.0, std::floor(width / std::max(rc, 1e-12))));
       cells = cells && nc[ax] >= 3;
     }
     std::vector<int> cellOf(n, 0), start;
