@@ -8,10 +8,11 @@
 // Weights are kept as FP32 masters (updated by Adam) and rounded to BF16 for the products; activations are stored
 // in BF16 for the next product and pre-activations in FP32 for the backward pass.
 //
-// A training step is seven kernels: the fused forward pass (features, three layers and the output for 16 rows per
+// A training step is five kernels: the fused forward pass (features, three layers and the output for 16 rows per
 // workgroup, activations in LDS), the loss, the fused backward pass down to the first layer's pre-activation
-// gradients, the bias and output-weight gradients as row sums, three weight-gradient products (64 x 64 tiles,
-// split K, operands stored feature-major so every MFMA operand is one 16-byte LDS read), and Adam.
+// gradients, the parameter gradients (three weight-gradient products in 64 x 64 tiles, split K, operands stored
+// feature-major so every MFMA operand is one 16-byte LDS read, and in the same launch the bias and output-weight
+// gradients as row sums), and Adam.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,24 +37,20 @@ constexpr int NIN = 6 * NFREQ;  // 192
 __device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // Weight gradient C (M x N) += A B^T over k, with A (M x K) and B (N x K) row-major (k contiguous: the layer inputs
-// and pre-activation gradients stored feature-major).  Workgroup (x, y, z) computes the 64 x 64 tile (y, x) over k in
-// [z Kc, min(K, (z + 1) Kc)) and adds it atomically (split K).  K is taken in stages of up to KS = 256: every global
-// load of a stage is issued before any of them is written to LDS, so a stage costs one memory latency.
+// and pre-activation gradients stored feature-major).  The workgroup computes the 64 x 64 tile (m0, n0) over k in
+// [kBegin, kEnd) and adds it atomically (split K).  K is taken in stages of up to KS = 256: every global load of a
+// stage is issued before any of them is written to LDS, so a stage costs one memory latency.
 constexpr int KS = 256;
-__global__ __launch_bounds__(256) void k_weight_grad(int M, int N, int K, int Kc, const __bf16* __restrict__ A,
-                                                     int lda, const __bf16* __restrict__ B, int ldb,
-                                                     float* __restrict__ C)
+__device__ __forceinline__ void weightGradTile(int N, int kBegin, int kEnd, const __bf16* __restrict__ A, int lda,
+                                               const __bf16* __restrict__ B, int ldb, float* __restrict__ C, int m0,
+                                               int n0, __bf16 (*As)[KS + 8], __bf16 (*Bs)[KS + 8])
 {
-  __shared__ __attribute__((aligned(16))) __bf16 As[64][KS + 8];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][KS + 8];
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int kEnd = min(K, int(blockIdx.z + 1) * Kc);
-  for (int k0 = blockIdx.z * Kc; k0 < kEnd; k0 += KS)
+  for (int k0 = kBegin; k0 < kEnd; k0 += KS)
   {
     const int kk = min(KS, kEnd - k0);  // a multiple of 32
     const int nv = 8 * kk;               // 8-element vectors per operand tile (64 rows x kk)
@@ -462,16 +459,14 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ 
 }
 
 // Bias gradients and the output weights' gradient as row sums over the batch, one wavefront per sum (no atomics, a
-// fixed order): db1, db2, db3 = sums of the rows of dZ1t, dZ2t, dZ3t; dw4 = rows of H3t weighted by df; db4 = sum
-// of df.  Every sum is stored, not added: the caller cleared g.
-__global__ __launch_bounds__(256) void k_bias_grads(int n, int ld, const __bf16* __restrict__ dZ1t,
-                                                    const __bf16* __restrict__ dZ2t, const __bf16* __restrict__ dZ3t,
-                                                    const __bf16* __restrict__ H3t, const float* __restrict__ df,
-                                                    float* __restrict__ g)
+// fixed order): task 0..4 WIDTH - 1 takes row j = task % WIDTH of dZ1t, dZ2t, dZ3t (db1, db2, db3) or of H3t
+// weighted by df (dw4); task 4 WIDTH sums df (db4).  Every sum is stored, not added: the caller cleared g.
+__device__ __forceinline__ void biasSum(int task, int lane, int n, int ld, const __bf16* __restrict__ dZ1t,
+                                        const __bf16* __restrict__ dZ2t, const __bf16* __restrict__ dZ3t,
+                                        const __bf16* __restrict__ H3t, const float* __restrict__ df,
+                                        float* __restrict__ g)
 {
-  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (wid > 4 * WIDTH) return;
-  const int which = wid / WIDTH, j = wid % WIDTH;
+  const int which = task / WIDTH, j = task % WIDTH;
   // n is a multiple of 16: each lane reads 8 consecutive values per step (16-byte loads, all in flight together)
   float s = 0.f;
   if (which == 4)
@@ -506,6 +501,40 @@ __global__ __launch_bounds__(256) void k_bias_grads(int n, int ld, const __bf16*
     const size_t off = which == 0 ? OFF_B1 : which == 1 ? OFF_B2 : which == 2 ? OFF_B3 : which == 3 ? OFF_W4 : OFF_B4;
     g[off + (which == 4 ? 0 : j)] = s;
   }
+}
+
+// The parameter gradients of a minibatch in one launch: z-slices 0 .. 3 nsplit - 1 are the three weight-gradient
+// products (layer l = z / nsplit, K chunk z % nsplit of Kc rows; 64 x 64 tiles, workgroups beyond a layer's M
+// return at once), the last BIAS_Z slices the row sums of biasSum spread over their wavefronts.
+constexpr int BIAS_Z = 4;
+struct ParamGradArgs
+{
+  const __bf16* A[3];  // layer inputs, feature-major: H2t, H1t, Xt
+  const __bf16* B[3];  // pre-activation gradients, feature-major: dZ3t, dZ2t, dZ1t
+  float* C[3];         // gradient blocks: W3, W2, W1
+  int M[3];            // rows of each block: WIDTH, WIDTH, NIN
+  const __bf16 *dZ1t, *dZ2t, *dZ3t, *H3t;
+  const float* df;
+  float* g;
+};
+__global__ __launch_bounds__(256) void k_param_grads(ParamGradArgs a, int K, int Kc, int nsplit, int ld)
+{
+  __shared__ __attribute__((aligned(16))) __bf16 As[64][KS + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[64][KS + 8];
+  const int z = blockIdx.z;
+  if (z >= 3 * nsplit)
+  {
+    const int lane = threadIdx.x & 63;
+    const int nw = 4 * gridDim.x * gridDim.y * BIAS_Z;
+    const int wid = 4 * ((z - 3 * nsplit) * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x) +
+                    int(threadIdx.x >> 6);
+    for (int task = wid; task <= 4 * WIDTH; task += nw) biasSum(task, lane, K, ld, a.dZ1t, a.dZ2t, a.dZ3t, a.H3t, a.df, a.g);
+    return;
+  }
+  const int l = z / nsplit, sp = z % nsplit;
+  if (int(blockIdx.y) * 64 >= a.M[l]) return;  // the whole workgroup leaves before any barrier
+  weightGradTile(WIDTH, sp * Kc, min(K, (sp + 1) * Kc), a.A[l], ld, a.B[l], ld, a.C[l], blockIdx.y * 64,
+                 blockIdx.x * 64, As, Bs);
 }
 
 __global__ void k_zero(float* p, int n)
@@ -635,21 +664,27 @@ rg_status forward(rg_mlp* p, int nrow, bool train, const int* perm = nullptr, co
   return RG_OK;
 }
 
-// Weight gradient G (M x WIDTH) += A dZ^T over n rows, with A (M x n) and dZ (WIDTH x n) feature-major (row stride
-// ld), split over K chunks of kc rows whose partial products are added atomically.  The chunk is tuned online among
-// 256 .. 4096 rows (results differ only in the order of the FP32 atomic additions, which is unordered anyway).
-rg_status weightGrad(rg_ctx* ctx, int M, int n, int ld, const __bf16* A, const __bf16* dZ, float* G)
+// The parameter gradients (k_param_grads) of n rows: the K chunk of the weight-gradient products is tuned online
+// among 256 .. 4096 rows (results differ only in the order of the FP32 atomic additions, which is unordered anyway).
+rg_status paramGrads(rg_mlp* p, int n)
 {
+  rg_ctx* ctx = p->ctx;
+  ParamGradArgs a{};
+  a.A[0] = p->H2t, a.B[0] = p->dZ3t, a.C[0] = p->g + OFF_W3, a.M[0] = WIDTH;
+  a.A[1] = p->H1t, a.B[1] = p->dZ2t, a.C[1] = p->g + OFF_W2, a.M[1] = WIDTH;
+  a.A[2] = p->Xt, a.B[2] = p->dZ1t, a.C[2] = p->g + OFF_W1, a.M[2] = NIN;
+  a.dZ1t = p->dZ1t, a.dZ2t = p->dZ2t, a.dZ3t = p->dZ3t, a.H3t = p->H3t, a.df = p->df, a.g = p->g;
   static const int kChunks[5] = {256, 512, 1024, 2048, 4096};
   auto launch = [&](int kc)
   {
     kc = std::min(kc, n);
-    const dim3 grid(WIDTH / 64, M / 64, (n + kc - 1) / kc);
-    hipLaunchKernelGGL(k_weight_grad, grid, dim3(256), 0, ctx->stream, M, WIDTH, n, kc, A, ld, dZ, ld, G);
+    const int nsplit = (n + kc - 1) / kc;
+    const dim3 grid(WIDTH / 64, WIDTH / 64, 3 * nsplit + BIAS_Z);
+    hipLaunchKernelGGL(k_param_grads, grid, dim3(256), 0, ctx->stream, a, n, kc, nsplit, p->cap);
     return hipGetLastError();
   };
   Autotuner& tuner = Autotuner::instance();
-  const std::string key = "mlp_weight_grad/M=" + std::to_string(M) + "/n=" + std::to_string(n);
+  const std::string key = "mlp_param_grads/n=" + std::to_string(n);
   const int cand = tuner.next(key, 5, 2);
   if (cand >= 0)
   {
@@ -663,19 +698,15 @@ rg_status weightGrad(rg_ctx* ctx, int M, int n, int ld, const __bf16* A, const _
 }
 
 // Loss and gradients (added into p->g, which the caller has cleared) of a batch of n rows (a multiple of FROWS)
-// whose forward pass ran with train = true: the loss kernel, the fused backward pass and the three weight gradients.
+// whose forward pass ran with train = true: the loss kernel, the fused backward pass and the parameter gradients.
 rg_status backward(rg_mlp* p, int n, double beta)
 {
   hipStream_t st = p->ctx->stream;
   hipLaunchKernelGGL(k_loss, dim3(1), dim3(LOSS_THREADS), 0, st, n, p->f, p->U, beta, p->df, p->loss);
   hipLaunchKernelGGL(k_mlp_backward, dim3(n / FROWS), dim3(256), 0, st, p->df, p->Z1, p->Z2, p->Z3, p->wb, p->dZ1t,
                      p->dZ2t, p->dZ3t, p->cap);
-  hipLaunchKernelGGL(k_bias_grads, dim3((4 * WIDTH + 1 + 3) / 4), dim3(256), 0, st, n, p->cap, p->dZ1t, p->dZ2t,
-                     p->dZ3t, p->H3t, p->df, p->g);
   rg_status ws;
-  if ((ws = weightGrad(p->ctx, WIDTH, n, p->cap, p->H2t, p->dZ3t, p->g + OFF_W3)) != RG_OK) return ws;
-  if ((ws = weightGrad(p->ctx, WIDTH, n, p->cap, p->H1t, p->dZ2t, p->g + OFF_W2)) != RG_OK) return ws;
-  if ((ws = weightGrad(p->ctx, NIN, n, p->cap, p->Xt, p->dZ1t, p->g + OFF_W1)) != RG_OK) return ws;
+  if ((ws = paramGrads(p, n)) != RG_OK) return ws;
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
 }
