@@ -40,7 +40,7 @@ struct alignas(16) Img
   double x, y, z, q;    // Cartesian position of the image, charge (e)
   double u0, u1, u2;    // u_a = h_a . x (lattice-vector projections, for triclinic derivatives)
   int type;             // local framework type row
-  int pad;
+  float qf;             // charge times the Coulomb factor in FP32 (filled when a launch stages the image)
 };
 static_assert(sizeof(Img) == 64, "Img must be 64 bytes");
 
@@ -355,6 +355,10 @@ __device__ __forceinline__ int stageChunk(const GridArgs& A, const TilePoint& T,
   if (pass)
   {
     r1.y *= A.coulomb;
+    // the record's last word (type, qf): keep the type, store the scaled charge in FP32
+    r3.y = __longlong_as_double(
+        (__double_as_longlong(r3.y) & 0xffffffffll) |
+        (static_cast<long long>(__float_as_uint(static_cast<float>(r1.y))) << 32));
     double2* dst = reinterpret_cast<double2*>(&cand[off + __popcll(m & lt_mask)]);
     dst[0] = r0;
     dst[1] = r1;
@@ -397,24 +401,21 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   {
     const int ncand = stageChunk<NLJ, COUL>(A, T, base, total, rowStart, rowPrefix, par, cand, waveCount);
 
-    // ---- per-lane pair sums over the staged candidates; the position and charge of the next candidate are read
-    // while the current one is summed
+    // ---- per-lane pair sums over the staged candidates (every lane reads the same record: an LDS broadcast).  The
+    // four waves of a workgroup hide each other's LDS latency; a register prefetch of the next record cost four
+    // moves per pair and ran slower.
     const double2* cb = reinterpret_cast<const double2*>(cand);
-    double2 nx0 = cb[0], nx1 = cb[1];
     for (int c = 0; c < ncand; ++c)
     {
-      const double2 c0 = nx0, c1 = nx1;  // (x, y), (z, q)
-      {
-        const int cn = min(c + 1, ncand - 1);
-        nx0 = cb[4 * cn];
-        nx1 = cb[4 * cn + 1];
-      }
+      const double2 c0 = cb[4 * c], c1 = cb[4 * c + 1];  // (x, y), (z, q)
       const Img& b = cand[c];
       const double dx = px - c0.x, dy = py - c0.y, dz = pz - c1.x;
-      double rr = dx * dx + dy * dy + dz * dz;
+      const double rr = dx * dx + dy * dy + dz * dz;
       if (rr < A.rc2)
       {
-        rr = fmax(rr, 0.1);  // src/raspakit/interactions/framework_molecule_grid.cpp:200
+        // src/raspakit/interactions/framework_molecule_grid.cpp:200; the FP32 path below takes rr itself, which the
+        // host's rsplit^2 >= 0.1 keeps above the clamp
+        const double rrc = fmax(rr, 0.1);
         double u0, u1, u2;
         if (TRIC)
         {
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
         const double u01 = u0 * u1, u02 = u0 * u2, u12 = u1 * u2, u012 = u01 * u2;
         if (NLJ > 0)
         {
-          const double ir2 = rcp64(rr);
+          const double ir2 = rcp64(rrc);
           const double ir4 = ir2 * ir2, ir6 = ir4 * ir2;
 #pragma unroll
           for (int q = 0; q < NLJ; ++q)
@@ -451,55 +452,60 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
             }
           }
         }
-        // a Coulomb-only launch stages charged images only
-        if (COUL && SPLIT && !__ballot(rr < A.rsplit2))
+        // a Coulomb-only launch stages charged images only.  Both precisions only form the radial factors; the one
+        // accumulation after the join keeps the accumulators in place (accumulating inside each branch made the
+        // compiler copy all twelve of them out and back around the join, 24 moves per pair).
+        if (COUL && (NLJ == 0 || c1.y != 0.0))
         {
-          // every lane of the wavefront is at least the split radius from this image: the same radial factors in
-          // FP32 (hardware rsq and exp2, erfcx from the fitted polynomial), accumulated in FP64 like the rest.  The
-          // exponent alpha^2 r^2 / ln 2 is reduced in FP64 so that only its fractional part passes through FP32.
-          const float rr32 = static_cast<float>(rr);
-          const float ir = __builtin_amdgcn_rsqf(rr32);
-          const float r = rr32 * ir;
-          const float t = fmaf(r, A.xs32, A.xo32);
-          float px = A.ecx[kSplitDeg];
+          double e, f1, f2, f3;
+          if (SPLIT && !__ballot(rr < A.rsplit2))
+          {
+            // every lane of the wavefront is at least the split radius from this image: the radial factors in FP32
+            // (hardware rsq and exp2, erfcx from the fitted polynomial).  The exponent -alpha^2 r^2 / ln 2 = k + f
+            // (k = its floor) is reduced in FP64 so that only f in [0, 1) passes through FP32.
+            const float rr32 = static_cast<float>(rr);
+            const float ir = __builtin_amdgcn_rsqf(rr32);
+            const float r = rr32 * ir;
+            const float t = fmaf(r, A.xs32, A.xo32);
+            float pe = A.ecx[kSplitDeg];
 #pragma unroll
-          for (int j = kSplitDeg - 1; j >= 0; --j) px = fmaf(px, t, A.ecx[j]);
-          const double y = rr * A.a2l2e;
-          const double fl = floor(y);
-          const float ex = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(fl - y)),
-                                                   -static_cast<int>(fl));
-          const float qex = static_cast<float>(c1.y) * ex;  // q exp(-alpha^2 r^2)
-          const float ec = qex * px;                         // q erfc(alpha r)
-          const float etr = A.k2a32 * r * qex;
-          const float ar2 = A.alpha2_32 * rr32;
-          const float ir2 = ir * ir;
-          const float ir3 = ir2 * ir;
-          const float ir5 = ir3 * ir2;
-          const float ir7 = ir5 * ir2;
-          const float e = ec * ir;
-          const float f1 = -(etr + ec) * ir3;
-          const float f2 = fmaf(etr, fmaf(2.0f, ar2, 3.0f), 3.0f * ec) * ir5;
-          const float f3 = -fmaf(etr, fmaf(ar2, fmaf(4.0f, ar2, 10.0f), 15.0f), 15.0f * ec) * ir7;
-          acc[NG - 1].add(double(e), double(f1), double(f2), double(f3), u0, u1, u2, u01, u02, u12, u012);
-        }
-        else if (COUL && (NLJ == 0 || c1.y != 0.0))
-        {
-          // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 with a unit probe charge;
-          // the staged charge already carries the Coulomb conversion factor.
-          const double ir = rsq64(rr);
-          const double r = rr * ir;
-          const double ir2 = ir * ir;
-          const double ir3 = ir2 * ir;
-          const double ir5 = ir3 * ir2;
-          const double ir7 = ir5 * ir2;
-          const double ar2 = A.alpha2 * rr;
-          const double qex = c1.y * expNeg(ar2);               // q exp(-alpha^2 r^2)
-          const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);  // q erfc(alpha r)
-          const double etr = A.k2a * r * qex;                 // q 2 alpha / sqrt(pi) r exp(-alpha^2 r^2)
-          const double e = ec * ir;
-          const double f1 = -(etr + ec) * ir3;
-          const double f2 = fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir5;
-          const double f3 = -fma(etr, fma(ar2, fma(4.0, ar2, 10.0), 15.0), 15.0 * ec) * ir7;
+            for (int j = kSplitDeg - 1; j >= 0; --j) pe = fmaf(pe, t, A.ecx[j]);
+            const double y = rr * -A.a2l2e;
+            const double k = floor(y);
+            const float ex = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(y - k)),
+                                                     static_cast<int>(k));
+            const float qex = b.qf * ex;                       // q exp(-alpha^2 r^2)
+            const float ec = qex * pe;                         // q erfc(alpha r)
+            const float etr = A.k2a32 * r * qex;
+            const float ar2 = A.alpha2_32 * rr32;
+            const float ir2 = ir * ir;
+            const float ir3 = ir2 * ir;
+            const float ir5 = ir3 * ir2;
+            const float ir7 = ir5 * ir2;
+            e = double(ec * ir);
+            f1 = double(-(etr + ec) * ir3);
+            f2 = double(fmaf(etr, fmaf(2.0f, ar2, 3.0f), 3.0f * ec) * ir5);
+            f3 = double(-fmaf(etr, fmaf(ar2, fmaf(4.0f, ar2, 10.0f), 15.0f), 15.0f * ec) * ir7);
+          }
+          else
+          {
+            // src/raspakit/potentials/interpolation/tricubic_derivative_real_ewald.ixx:32-56 with a unit probe
+            // charge; the staged charge already carries the Coulomb conversion factor.
+            const double ir = rsq64(rrc);
+            const double r = rrc * ir;
+            const double ir2 = ir * ir;
+            const double ir3 = ir2 * ir;
+            const double ir5 = ir3 * ir2;
+            const double ir7 = ir5 * ir2;
+            const double ar2 = A.alpha2 * rrc;
+            const double qex = c1.y * expNeg(ar2);                   // q exp(-alpha^2 r^2)
+            const double ec = qex * erfcxLds(A.alpha * r, erfcxTab);  // q erfc(alpha r)
+            const double etr = A.k2a * r * qex;                     // q 2 alpha / sqrt(pi) r exp(-alpha^2 r^2)
+            e = ec * ir;
+            f1 = -(etr + ec) * ir3;
+            f2 = fma(etr, fma(2.0, ar2, 3.0), 3.0 * ec) * ir5;
+            f3 = -fma(etr, fma(ar2, fma(4.0, ar2, 10.0), 15.0), 15.0 * ec) * ir7;
+          }
           acc[NG - 1].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
         }
       }
@@ -597,15 +603,9 @@ __global__ __launch_bounds__(256) void k_grid_build_q(GridArgs A)
   {
     const int ncand = stageChunk<NLJ, COUL>(A, T, base, total, rowStart, rowPrefix, par, cand, waveCount);
     const double2* cb = reinterpret_cast<const double2*>(cand);
-    double2 nx0 = cb[0], nx1 = cb[1];
     for (int c = 0; c < ncand; ++c)
     {
-      const double2 c0 = nx0, c1 = nx1;  // (x, y), (z, q)
-      {
-        const int cn = min(c + 1, ncand - 1);
-        nx0 = cb[4 * cn];
-        nx1 = cb[4 * cn + 1];
-      }
+      const double2 c0 = cb[4 * c], c1 = cb[4 * c + 1];  // (x, y), (z, q): an LDS broadcast
       const Img& b = cand[c];
       const double dx = T.px - c0.x, dy = T.py - c0.y, dz = T.pz - c1.x;
       double rr = dx * dx + dy * dy + dz * dz;
@@ -1002,6 +1002,9 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
       return rg::fail(ctx, RG_ERR_UNSUPPORTED,
                       "rg_grid_build: the FP32 erfcx polynomial reaches only %.2e relative on alpha r in [%g, %g]; "
                       "choose a larger split radius", err, ff->alpha * splitRadius, ff->alpha * rc);
+    if (splitRadius * splitRadius < 0.1)
+      return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: a split radius below sqrt(0.1) A lies inside the overlap "
+                      "clamp of the pair distance");
     A.rsplit2 = splitRadius * splitRadius;
     A.a2l2e = ff->alpha * ff->alpha / M_LN2;
     A.xs32 = static_cast<float>(ff->alpha * xs);
