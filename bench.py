@@ -67,11 +67,11 @@ FLOP_EWALD_QUINTIC = 223
 GRID_BYTES_PER_SITE = 512      # 8 corner chunks of 64 B per tricubic interpolation
 MLP_FLOP_PER_SAMPLE = 1.08e6   # forward 2 * 180.5k + backward 4 * 180.5k (GEMM work of a training sample)
 # HBM bytes per launch of the split-precision Ewald grid kernel on the headline workload, from two rocprofv3 --pmc
-# runs of "bench.py --grids-only --steps 1 --warmup 1 --no-cpu-baseline" (FETCH_SIZE 25,754 KiB, WRITE_SIZE
-# 2,910,675 KiB per dispatch), each scaled by what the same counter reads for a 1 GiB device-to-device copy ("bench.py
+# runs of "bench.py --grids-only --steps 1 --warmup 1 --no-cpu-baseline" (FETCH_SIZE 25,752 KiB, WRITE_SIZE
+# 2,910,675 KiB for the first dispatch), each scaled by what the same counter reads for a 1 GiB device-to-device copy ("bench.py
 # --copy-calibration 1024": FETCH_SIZE 524,300 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0).  The algorithmic
 # bytes are the 64 B written per point (2.98e9) and the 1.2 MB image list.
-EWALD_GRID_HBM_BYTES = 25754 * 1024 * 2.0 + 2910675 * 1024 * 1.0
+EWALD_GRID_HBM_BYTES = 25752 * 1024 * 2.0 + 2910675 * 1024 * 1.0
 
 
 def log(*a):
