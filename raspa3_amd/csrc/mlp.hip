@@ -859,8 +859,10 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
   if (e == hipSuccess)
     e = hipMemcpyAsync(dfrac, frac, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dU, energy, size_t(n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-  std::vector<int> perm(n);
-  for (int i = 0; i < n; ++i) perm[i] = i;
+  // epoch ep trains on the ep + 1 times shuffled identity; the next epoch's permutation is shuffled on the host (into
+  // the other of two host arrays) while the current epoch's kernels run
+  std::vector<int> perm[2] = {std::vector<int>(n), std::vector<int>()};
+  for (int i = 0; i < n; ++i) perm[0][i] = i;
   unsigned long long s = seed ? seed : 1;
   auto rnd = [&]()
   {
@@ -869,16 +871,22 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
     s ^= s << 17;
     return s;
   };
+  auto shuffle = [&](std::vector<int>& v)
+  {
+    for (int i = n - 1; i > 0; --i) std::swap(v[i], v[rnd() % (unsigned long long)(i + 1)]);
+  };
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const int nb = n / batch;
   // the gradient sums start from zero; k_adam clears them again after every step
   hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->g, int(NPARAM));
+  if (epochs > 0)
+  {
+    shuffle(perm[0]);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dperm, perm[0].data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+  }
   for (int ep = 0; ep < epochs && e == hipSuccess && st == RG_OK; ++ep)
   {
-    for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[rnd() % (unsigned long long)(i + 1)]);
-    // the previous epoch's kernels read dperm: the copy is ordered after them on the same stream
-    e = hipMemcpyAsync(dperm, perm.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) break;
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, ctx->stream, p->loss, 2);
     for (int b = 0; b < nb; ++b)
     {
@@ -891,6 +899,15 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
                          p->v, p->wb, p->wt, float(lr), b1, b2, eps, c1, c2);
     }
     e = hipGetLastError();
+    if (e == hipSuccess && st == RG_OK && ep + 1 < epochs)
+    {
+      // the host array of the next epoch was last copied before the previous epoch's synchronisation; the device
+      // copy is ordered after this epoch's kernels, which read dperm
+      std::vector<int>& next = perm[(ep + 1) & 1];
+      next = perm[ep & 1];
+      shuffle(next);
+      e = hipMemcpyAsync(dperm, next.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
+    }
     double l[2] = {0.0, 0.0};
     if (e == hipSuccess) e = hipMemcpyAsync(l, p->loss, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
