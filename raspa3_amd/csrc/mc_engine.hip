@@ -377,24 +377,6 @@ __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const
                                  const double* pos, int own, int skip, int lane, WaveScratch& W, const Team& T,
                                  double& Uj, bool& ovj)
 {
-  // grid-typed sites: lane j interpolates the sites of trial set j, so the K sets' gathers are in flight together
-  double gpart = 0.0;
-  bool gov = false;
-  if (lane < K && S.gridOfType)
-  {
-    for (int s = 0; s < n; ++s)
-    {
-      const int a = atomIdx[s];
-      const int gi = S.gridOfType[c.type[a]];
-      if (gi < 0) continue;
-      const double* p = pos + 3 * (lane * n + s);
-      const double g = rgd::interpolate(S.grids[gi], p[0], p[1], p[2]);
-      gov = gov || g > S.overlap;
-      gpart += g;
-      if (S.useCharge && S.coulGrid >= 0) gpart += c.q[a] * rgd::interpolate(S.grids[S.coulGrid], p[0], p[1], p[2]);
-    }
-  }
-  // adsorbate atoms (and framework atoms of types without a grid): all K * n sites in one batch
   const int ns = K * n;
   for (int s = 0; s < n; ++s)  // uniform loop: atomIdx stays in registers
   {
@@ -406,6 +388,51 @@ __device__ void trialSetEnergies(const SysDev& S, int nads, const Comp& c, const
     }
   }
   waveSyncLds();
+  // grid-typed sites: term t = 2 (j n + s) + g is term g of site s of trial set j (g = 0 the site type's grid, g = 1
+  // its charge times the Coulomb grid); lane t % 64 of round t / 64 takes it, so a round's gathers are all in flight
+  // at once.  Lane j adds its set's terms in increasing t (site by site, own grid before Coulomb: the order of one
+  // lane interpolating its whole set), whatever round they fall in.
+  double gpart = 0.0;
+  bool gov = false;
+  if (S.gridOfType)
+  {
+    const int nterm = 2 * ns;
+    const int tj = 2 * n * lane;  // lane j's first term
+    for (int r0 = 0; r0 < nterm; r0 += 64)
+    {
+      const int t = r0 + lane;
+      double v = 0.0;
+      bool o = false;
+      if (t < nterm)
+      {
+        const int site = t >> 1;
+        const bool coulTerm = (t & 1) != 0;
+        const int gi = S.gridOfType[W.sType[site]];
+        const int gsel = coulTerm ? ((S.useCharge && S.coulGrid >= 0) ? S.coulGrid : -1) : gi;
+        if (gi >= 0 && gsel >= 0)
+        {
+          const double* p = pos + 3 * site;
+          v = rgd::interpolate(S.grids[gsel], p[0], p[1], p[2]);
+          if (coulTerm)
+            v *= W.sQ[site];
+          else
+            o = v > S.overlap;
+        }
+      }
+      const unsigned long long om = __ballot(o);
+      for (int q = 0; q < 2 * n; ++q)
+      {
+        const int src = tj + q - r0;
+        const double x = __shfl(v, min(max(src, 0), 63), 64);
+        if (lane < K && src >= 0 && src < 64)
+        {
+          gpart += x;
+          gov = gov || ((om >> src) & 1ull) != 0ull;
+        }
+      }
+    }
+  }
+  // adsorbate atoms (and framework atoms of types without a grid): all K * n sites in one batch
   siteSetEnergies(S, nads, ns, pos, own, skip, lane, W, T);
   Uj = 0.0;
   ovj = false;
@@ -1338,19 +1365,39 @@ __device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const Sy
       W.pos[3 * (cm.na + lane) + 2] = nz;
     }
     waveSyncLds();
-    // grid parts of the old (lanes 0 .. na-1) and new (lanes na .. 2na-1) sites, interpolated concurrently
+    // grid parts of the old (sites 0 .. na-1) and new (sites na .. 2na-1) sites: lane 2 site + g takes term g of the
+    // site (g = 0 its type's grid, g = 1 its charge times the Coulomb grid), so all gathers are in flight at once;
+    // lane 'site' then adds its two terms
     double gl = 0.0;
     bool gov = false;
-    if (lane < 2 * cm.na && S.gridOfType)
+    if (S.gridOfType)
     {
-      const int a = lane < cm.na ? lane : lane - cm.na;
-      const int gi = S.gridOfType[cm.type[a]];
-      if (gi >= 0)
+      double v = 0.0;
+      bool o = false;
+      if (lane < 4 * cm.na)
       {
-        const double* p = W.pos + 3 * lane;
-        gl = rgd::interpolate(S.grids[gi], p[0], p[1], p[2]);
-        gov = gl > S.overlap;
-        if (S.useCharge && S.coulGrid >= 0) gl += cm.q[a] * rgd::interpolate(S.grids[S.coulGrid], p[0], p[1], p[2]);
+        const int site = lane >> 1;
+        const bool coulTerm = (lane & 1) != 0;
+        const int a = site < cm.na ? site : site - cm.na;
+        const int gi = S.gridOfType[cm.type[a]];
+        const int gsel = coulTerm ? ((S.useCharge && S.coulGrid >= 0) ? S.coulGrid : -1) : gi;
+        if (gi >= 0 && gsel >= 0)
+        {
+          const double* p = W.pos + 3 * site;
+          v = rgd::interpolate(S.grids[gsel], p[0], p[1], p[2]);
+          if (coulTerm)
+            v *= cm.q[a];
+          else
+            o = v > S.overlap;
+        }
+      }
+      const int src = min(2 * lane, 62);
+      const double own = __shfl(v, src, 64), coul = __shfl(v, src + 1, 64);
+      const bool oo = __shfl(o ? 1 : 0, src, 64) != 0;
+      if (lane < 2 * cm.na)
+      {
+        gl = own + coul;
+        gov = oo;
       }
     }
     const bool newLane = lane >= cm.na && lane < 2 * cm.na;
