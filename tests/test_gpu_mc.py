@@ -239,6 +239,27 @@ def test_gcmc_co2_replays_oracle_trajectory(gpu_ctx, scheme):
     mc.close()
 
 
+def test_gcmc_many_trials_replays_oracle_trajectory(gpu_ctx):
+    """Trial sets with more grid terms than lanes: 40 first-bead trials (80 grid terms: the site's grid and the
+    Coulomb grid per trial) and 20 orientations of CO2's two other atoms (80 terms), so each set's gathers take two
+    rounds of 64 lanes.  Two systems in charged ITQ-29, Ewald kmax 6, 3 + 6 cycles against the oracle's replay."""
+    print("case: gcmc CO2 replay, 40 first-bead trials and 20 orientations", flush=True)
+    spec = w.forcefield_zeolite()
+    types = [spec.index("C_co2"), spec.index("O_co2")]
+    fw, unit, ff, f, dims, grids, coul, host, hcoul = itq29_setup(gpu_ctx, spec, types)
+    comp = w.co2_reference(spec)
+    params = w.mc_params(pressure_pa=1e5, seed=11, k_first=40, k_orient=20, max_molecules=128, production_cycles=6)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, fw.rect, framework=f, grids=grids, coulomb_grid=coul, kmax=(6, 6, 6),
+                         capacity=8 * 128) for _ in range(2)]
+    mc = ra.Mc(gpu_ctx, systems, [comp], params)
+    mc.run(3, sample=False, graph=False)
+    mc.run(6, sample=True, graph=True)
+    for sidx in range(2):
+        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (6, 6, 6), host, hcoul, dims, unit, (3, 6))
+        assert g["accepted"][0] > 0
+    mc.close()
+
+
 def test_gcmc_reinsertion_replays_oracle_trajectory(gpu_ctx):
     """CBMC reinsertion (MC_Moves::reinsertionMove, src/raspakit/mc_moves/component/reinsertion.cpp:34-214) on the
     device against the oracle's replay: CO2/N2 80:20 in charged ITQ-29 with Ewald kmax 6, 30 % reinsertions,
