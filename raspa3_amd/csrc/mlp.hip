@@ -8,8 +8,8 @@
 // Weights are kept as FP32 masters (updated by Adam) and rounded to BF16 for the products; activations are stored
 // in BF16 for the next product and pre-activations in FP32 for the backward pass.
 //
-// A training step is five kernels: the fused forward pass (features, three layers and the output for 16 rows per
-// workgroup, activations in LDS), the loss, the fused backward pass down to the first layer's pre-activation
+// A training step is four kernels: the fused forward pass (features, three layers and the output for 16 rows per
+// workgroup, activations in LDS), the fused loss and backward pass down to the first layer's pre-activation
 // gradients, the parameter gradients (three weight-gradient products in 64 x 64 tiles, split K, operands stored
 // feature-major so every MFMA operand is one 16-byte LDS read, and in the same launch the bias and output-weight
 // gradients as row sums), and Adam.
@@ -194,10 +194,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const double* __restrict__ 
                                                      const float* __restrict__ w, float* __restrict__ Z1,
                                                      float* __restrict__ Z2, float* __restrict__ Z3, __bf16* __restrict__ Xt,
                                                      __bf16* __restrict__ H1t, __bf16* __restrict__ H2t,
-                                                     __bf16* __restrict__ H3t, int ldt, float* __restrict__ f)
+                                                     __bf16* __restrict__ H3t, int ldt, float* __restrict__ f,
+                                                     double beta, double* __restrict__ part)
 {
   __shared__ __attribute__((aligned(16))) __bf16 bufA[FROWS][LDSW];
   __shared__ __attribute__((aligned(16))) __bf16 bufB[FROWS][LDSW];
+  __shared__ double us[FROWS];  // -beta U of the rows (training)
+  __shared__ float fs[FROWS];
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int row0 = blockIdx.x * FROWS;
@@ -208,7 +211,12 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const double* __restrict__ 
     const int row = t & (FROWS - 1), ax = t / FROWS;
     const int gi = row0 + row;
     const int src = perm ? perm[gi] : gi;
-    if (TRAIN && perm && ax == 0) bU[gi] = Uall[src];
+    if (TRAIN && ax == 0)
+    {
+      const double u = perm ? Uall[src] : bU[gi];
+      if (perm) bU[gi] = u;
+      us[row] = -beta * u;
+    }
     double s1, c1;
     sincos(2.0 * M_PI * frac[3 * size_t(src) + ax], &s1, &c1);
     double sp = 0.0, cp = 1.0, sm = s1, cm = c1;
@@ -281,111 +289,115 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const double* __restrict__ 
     for (int k = FPT * part; k < FPT * part + FPT; ++k) s += float(in[row][k]) * float(wb[OFF_W4 + k]);
 #pragma unroll
     for (int o = 1; o < TPR; o <<= 1) s += __shfl_xor(s, o, 64);
-    if (part == 0) f[row0 + row] = s + w[OFF_B4];
+    if (part == 0)
+    {
+      f[row0 + row] = s + w[OFF_B4];
+      fs[row] = s + w[OFF_B4];
+    }
+  }
+  if (TRAIN)
+  {
+    // the workgroup's partial softmax statistics (the backward pass combines them): max f, sum exp(f - max f),
+    // max u, sum exp(u - max u) and sum exp(u - max u) f over its rows, u = -beta U
+    __syncthreads();
+    if (t < 64)
+    {
+      const bool live = t < FROWS;
+      const double fv = live ? double(fs[t]) : -INFINITY, uv = live ? us[t] : -INFINITY;
+      double mf = fv, mu = uv;
+      for (int o = 1; o < 64; o <<= 1)
+      {
+        mf = fmax(mf, __shfl_xor(mf, o, 64));
+        mu = fmax(mu, __shfl_xor(mu, o, 64));
+      }
+      const double eu = live ? double(expf(float(uv - mu))) : 0.0;
+      double sf = live ? double(expf(float(fv - mf))) : 0.0, su = eu, tu = eu * (live ? fv : 0.0);
+      for (int o = 1; o < 64; o <<= 1)
+      {
+        sf += __shfl_xor(sf, o, 64);
+        su += __shfl_xor(su, o, 64);
+        tu += __shfl_xor(tu, o, 64);
+      }
+      if (t == 0)
+      {
+        double* q = part + 5 * size_t(blockIdx.x);
+        q[0] = mf;
+        q[1] = sf;
+        q[2] = mu;
+        q[3] = su;
+        q[4] = tu;
+      }
+    }
   }
 }
 
-// One workgroup: p = softmax(-beta U), q = softmax(f), df = q - p, loss = -sum p log q.  The maxima and the
-// exponent arguments are formed in FP64, the exponentials evaluated in FP32 (df is FP32), the sums accumulated in FP64.
-// Batches of up to LOSS_CACHE * LOSS_THREADS rows are read once into registers; larger ones are re-read per pass.
-constexpr int LOSS_THREADS = 1024, LOSS_CACHE = 8;
-__global__ __launch_bounds__(LOSS_THREADS) void k_loss(int n, const float* __restrict__ f, const double* __restrict__ U,
-                                               double beta, float* __restrict__ df, double* __restrict__ loss)
+// Loss of a batch: p = softmax(u), u = -beta U, q = softmax(f), L = -sum p log q, dL/df = q - p; the maxima and
+// the exponent arguments in FP64, the exponentials of rows in FP32 (df is FP32), the sums in FP64.  Every workgroup
+// of the forward pass leaves partial statistics of its rows (max f, sum exp(f - max f), max u, sum exp(u - max u),
+// sum exp(u - max u) f); every workgroup of the backward pass combines all of them in the same order (so every
+// workgroup obtains the same values): M = max of the maxima, S = sum of the partial sums times exp(max - M) in FP64.
+// Then lse = Mf + log Sf and L = lse - T / Su, T the combined sum exp(u - Mu) f.
+__device__ __forceinline__ void reduceBlock3(double& a, double& b, double& c, bool isMax, double (*red)[4])
 {
-  __shared__ double red[2][16];
-  const int t = threadIdx.x, nt = blockDim.x;
-  auto reduce2 = [&](double& a, double& b, bool isMax)
+  const int t = threadIdx.x;
+  for (int o = 32; o > 0; o >>= 1)
   {
-    for (int o = 32; o > 0; o >>= 1)
-    {
-      const double x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64);
-      a = isMax ? fmax(a, x) : a + x;
-      b = isMax ? fmax(b, y) : b + y;
-    }
-    if ((t & 63) == 0)
-    {
-      red[0][t >> 6] = a;
-      red[1][t >> 6] = b;
-    }
-    __syncthreads();
-    a = red[0][0];
-    b = red[1][0];
-    for (int i = 1; i < (nt >> 6); ++i)
-    {
-      a = isMax ? fmax(a, red[0][i]) : a + red[0][i];
-      b = isMax ? fmax(b, red[1][i]) : b + red[1][i];
-    }
-    __syncthreads();
-  };
-  double mf = -INFINITY, mp = -INFINITY, sf = 0.0, sp = 0.0, l = 0.0, unused = 0.0;
-  if (n <= LOSS_CACHE * nt)
-  {
-    float fc[LOSS_CACHE];
-    double uc[LOSS_CACHE];  // -beta U
-#pragma unroll
-    for (int k = 0; k < LOSS_CACHE; ++k)
-    {
-      const int i = t + k * nt;
-      fc[k] = i < n ? f[i] : 0.f;
-      uc[k] = i < n ? -beta * U[i] : 0.0;
-      if (i < n)
-      {
-        mf = fmax(mf, double(fc[k]));
-        mp = fmax(mp, uc[k]);
-      }
-    }
-    reduce2(mf, mp, true);
-#pragma unroll
-    for (int k = 0; k < LOSS_CACHE; ++k)
-      if (t + k * nt < n)
-      {
-        sf += expf(float(double(fc[k]) - mf));
-        sp += expf(float(uc[k] - mp));
-      }
-    reduce2(sf, sp, false);
-    const double lse = mf + log(sf);
-#pragma unroll
-    for (int k = 0; k < LOSS_CACHE; ++k)
-    {
-      const int i = t + k * nt;
-      if (i < n)
-      {
-        const double p = double(expf(float(uc[k] - mp))) / sp;
-        const double lq = double(fc[k]) - lse;
-        df[i] = float(double(expf(float(lq))) - p);
-        l -= p * lq;
-      }
-    }
+    const double x = __shfl_xor(a, o, 64), y = __shfl_xor(b, o, 64), z = __shfl_xor(c, o, 64);
+    a = isMax ? fmax(a, x) : a + x;
+    b = isMax ? fmax(b, y) : b + y;
+    c = isMax ? fmax(c, z) : c + z;
   }
-  else
+  if ((t & 63) == 0)
   {
-    for (int i = t; i < n; i += nt)
-    {
-      mf = fmax(mf, double(f[i]));
-      mp = fmax(mp, -beta * U[i]);
-    }
-    reduce2(mf, mp, true);
-    for (int i = t; i < n; i += nt)
-    {
-      sf += expf(float(double(f[i]) - mf));
-      sp += expf(float(-beta * U[i] - mp));
-    }
-    reduce2(sf, sp, false);
-    const double lse = mf + log(sf);
-    for (int i = t; i < n; i += nt)
-    {
-      const double p = double(expf(float(-beta * U[i] - mp))) / sp;
-      const double lq = double(f[i]) - lse;
-      df[i] = float(double(expf(float(lq))) - p);
-      l -= p * lq;
-    }
+    red[0][t >> 6] = a;
+    red[1][t >> 6] = b;
+    red[2][t >> 6] = c;
   }
-  reduce2(l, unused, false);
-  if (t == 0)
+  __syncthreads();
+  a = red[0][0];
+  b = red[1][0];
+  c = red[2][0];
+  for (int i = 1; i < 4; ++i)
   {
-    loss[0] = l;
-    loss[1] += l;
+    a = isMax ? fmax(a, red[0][i]) : a + red[0][i];
+    b = isMax ? fmax(b, red[1][i]) : b + red[1][i];
+    c = isMax ? fmax(c, red[2][i]) : c + red[2][i];
   }
+  __syncthreads();
+}
+
+struct LossStats
+{
+  double mf, mu, su, lse, loss;
+};
+
+__device__ __forceinline__ LossStats combineStats(int nparts, const double* __restrict__ part, double (*red)[4])
+{
+  const int t = threadIdx.x;
+  double mf = -INFINITY, mu = -INFINITY, unused = 0.0;
+  for (int j = t; j < nparts; j += 256)
+  {
+    mf = fmax(mf, part[5 * j]);
+    mu = fmax(mu, part[5 * j + 2]);
+  }
+  reduceBlock3(mf, mu, unused, true, red);
+  double sf = 0.0, su = 0.0, tu = 0.0;
+  for (int j = t; j < nparts; j += 256)
+  {
+    const double* q = part + 5 * j;
+    const double a = exp(q[0] - mf), b = exp(q[2] - mu);
+    sf += q[1] * a;
+    su += q[3] * b;
+    tu += q[4] * b;
+  }
+  reduceBlock3(sf, su, tu, false, red);
+  LossStats L;
+  L.mf = mf;
+  L.mu = mu;
+  L.su = su;
+  L.lse = mf + log(sf);
+  L.loss = L.lse - tu / su;
+  return L;
 }
 
 // Backward through one hidden layer for the workgroup's rows: dH^T = W dZ^T (W row-major in x out, dZ the LDS tile),
@@ -424,18 +436,42 @@ __device__ __forceinline__ void backLayer(const __bf16* __restrict__ W, const fl
 // Backward pass of FROWS rows per workgroup, down to the gradient of the first layer's pre-activations: output layer
 // (dZ3 = df w4 SiLU'(Z3)), then layers 3 and 2 (dZ2, dZ1), all stored feature-major (ld = ldt) for the parameter
 // gradients.
-__global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ df, const float* __restrict__ Z1,
-                                                      const float* __restrict__ Z2, const float* __restrict__ Z3,
-                                                      const __bf16* __restrict__ wb, __bf16* __restrict__ dZ1t,
-                                                      __bf16* __restrict__ dZ2t, __bf16* __restrict__ dZ3t, int ldt)
+// The loss and the backward pass of n rows: every workgroup first combines the forward pass's partial statistics,
+// then forms the loss gradient dL/df = q - p of its own FROWS rows (stored to df for the parameter gradients) and
+// propagates it down to the first layer's pre-activation gradients.  Workgroup 0 also stores the batch loss
+// (loss[0] = L, loss[1] += L).
+__global__ __launch_bounds__(256) void k_mlp_backward(int n, const float* __restrict__ f, const double* __restrict__ U,
+                                                      double beta, const double* __restrict__ part,
+                                                      float* __restrict__ df, double* __restrict__ loss,
+                                                      const float* __restrict__ Z1, const float* __restrict__ Z2,
+                                                      const float* __restrict__ Z3, const __bf16* __restrict__ wb,
+                                                      __bf16* __restrict__ dZ1t, __bf16* __restrict__ dZ2t,
+                                                      __bf16* __restrict__ dZ3t, int ldt)
 {
   __shared__ __attribute__((aligned(16))) __bf16 bufA[FROWS][LDSW];
   __shared__ __attribute__((aligned(16))) __bf16 bufB[FROWS][LDSW];
   __shared__ float dfs[FROWS];
+  __shared__ double red[3][4];
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int row0 = blockIdx.x * FROWS;
-  if (t < FROWS) dfs[t] = df[row0 + t];
+  {
+    const LossStats L = combineStats(n / FROWS, part, red);
+    if (t < FROWS)
+    {
+      const int i = row0 + t;
+      const double p = double(expf(float(-beta * U[i] - L.mu))) / L.su;
+      const double lq = double(f[i]) - L.lse;
+      const float d = float(double(expf(float(lq))) - p);
+      dfs[t] = d;
+      df[i] = d;
+    }
+    if (blockIdx.x == 0 && t == 0)
+    {
+      loss[0] = L.loss;
+      loss[1] += L.loss;
+    }
+  }
   __syncthreads();
   {
     // output layer, one thread per hidden unit j
@@ -602,6 +638,7 @@ struct rg_mlp
          *dZ3t = nullptr;
   float *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr, *f = nullptr, *df = nullptr;
   double *frac = nullptr, *U = nullptr, *loss = nullptr;
+  double* part = nullptr;  // the forward pass's partial softmax statistics, 5 per workgroup
 };
 
 namespace
@@ -610,12 +647,12 @@ namespace
 void freeBatch(rg_mlp* p)
 {
   void* bufs[] = {p->Xt, p->H1t, p->H2t, p->H3t, p->dZ1t, p->dZ2t, p->dZ3t, p->Z1, p->Z2, p->Z3, p->f, p->df,
-                  p->frac, p->U, p->loss};
+                  p->frac, p->U, p->loss, p->part};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   p->Xt = p->H1t = p->H2t = p->H3t = p->dZ1t = p->dZ2t = p->dZ3t = nullptr;
   p->Z1 = p->Z2 = p->Z3 = p->f = p->df = nullptr;
-  p->frac = p->U = p->loss = nullptr;
+  p->frac = p->U = p->loss = p->part = nullptr;
   p->cap = 0;
 }
 
@@ -642,24 +679,27 @@ rg_status ensureBatch(rg_mlp* p, int n)
   RG_HIP(ctx, hipMalloc(&p->frac, nb * 3 * sizeof(double)));
   RG_HIP(ctx, hipMalloc(&p->U, nb * sizeof(double)));
   RG_HIP(ctx, hipMalloc(&p->loss, 8 * sizeof(double)));
+  RG_HIP(ctx, hipMalloc(&p->part, nb / FROWS * 5 * sizeof(double)));
   p->cap = need;
   return RG_OK;
 }
 
 // Forward pass of nrow rows (a multiple of FROWS, at most the batch capacity): fractional coordinates from p->frac,
 // or (perm != nullptr) gathered from fracAll through perm with the energies copied from Uall to p->U.  train keeps
-// what the backward pass needs.
-rg_status forward(rg_mlp* p, int nrow, bool train, const int* perm = nullptr, const double* fracAll = nullptr,
-                  const double* Uall = nullptr)
+// what the backward pass needs, including the partial loss statistics at inverse temperature beta.
+rg_status forward(rg_mlp* p, int nrow, bool train, double beta = 0.0, const int* perm = nullptr,
+                  const double* fracAll = nullptr, const double* Uall = nullptr)
 {
   if (nrow % FROWS != 0 || nrow > p->cap) return fail(p->ctx, RG_ERR_INVALID, "mlp forward: %d rows", nrow);
   const double* fr = perm ? fracAll : p->frac;
   if (train)
     hipLaunchKernelGGL(k_mlp_forward<true>, dim3(nrow / FROWS), dim3(256), 0, p->ctx->stream, fr, perm, Uall, p->U,
-                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f);
+                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f, beta,
+                       p->part);
   else
     hipLaunchKernelGGL(k_mlp_forward<false>, dim3(nrow / FROWS), dim3(256), 0, p->ctx->stream, fr, perm, Uall, p->U,
-                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f);
+                       p->wt, p->wb, p->w, p->Z1, p->Z2, p->Z3, p->Xt, p->H1t, p->H2t, p->H3t, p->cap, p->f, beta,
+                       nullptr);
   RG_HIP(p->ctx, hipGetLastError());
   return RG_OK;
 }
@@ -698,13 +738,12 @@ rg_status paramGrads(rg_mlp* p, int n)
 }
 
 // Loss and gradients (added into p->g, which the caller has cleared) of a batch of n rows (a multiple of FROWS)
-// whose forward pass ran with train = true: the loss kernel, the fused backward pass and the parameter gradients.
+// whose forward pass ran with train = true: the backward pass with the loss fused in, then the parameter gradients.
 rg_status backward(rg_mlp* p, int n, double beta)
 {
   hipStream_t st = p->ctx->stream;
-  hipLaunchKernelGGL(k_loss, dim3(1), dim3(LOSS_THREADS), 0, st, n, p->f, p->U, beta, p->df, p->loss);
-  hipLaunchKernelGGL(k_mlp_backward, dim3(n / FROWS), dim3(256), 0, st, p->df, p->Z1, p->Z2, p->Z3, p->wb, p->dZ1t,
-                     p->dZ2t, p->dZ3t, p->cap);
+  hipLaunchKernelGGL(k_mlp_backward, dim3(n / FROWS), dim3(256), 0, st, n, p->f, p->U, beta, p->part, p->df, p->loss,
+                     p->Z1, p->Z2, p->Z3, p->wb, p->dZ1t, p->dZ2t, p->dZ3t, p->cap);
   rg_status ws;
   if ((ws = paramGrads(p, n)) != RG_OK) return ws;
   RG_HIP(p->ctx, hipGetLastError());
@@ -833,7 +872,7 @@ rg_status rg_mlp_gradients(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, co
   RG_HIP(ctx, hipMemcpyAsync(p->frac, frac, size_t(n) * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(p->U, energy, size_t(n) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   hipLaunchKernelGGL(k_zero, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, p->g, int(NPARAM));
-  if ((st = forward(p, n, true)) != RG_OK) return st;
+  if ((st = forward(p, n, true, beta)) != RG_OK) return st;
   if ((st = backward(p, n, beta)) != RG_OK) return st;
   RG_HIP(ctx, hipMemcpyAsync(grads, p->g, NPARAM * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
   RG_HIP(ctx, hipMemcpyAsync(loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -890,7 +929,7 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
     hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, ctx->stream, p->loss, 2);
     for (int b = 0; b < nb; ++b)
     {
-      if ((st = forward(p, batch, true, dperm + size_t(b) * batch, dfrac, dU)) != RG_OK ||
+      if ((st = forward(p, batch, true, beta, dperm + size_t(b) * batch, dfrac, dU)) != RG_OK ||
           (st = backward(p, batch, beta)) != RG_OK)
         break;
       p->adamStep += 1;
