@@ -310,8 +310,9 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const double* __restrict__ 
         mf = fmax(mf, __shfl_xor(mf, o, 64));
         mu = fmax(mu, __shfl_xor(mu, o, 64));
       }
-      const double eu = live ? double(expf(float(uv - mu))) : 0.0;
-      double sf = live ? double(expf(float(fv - mf))) : 0.0, su = eu, tu = eu * (live ? fv : 0.0);
+      // rows of infinite energy (u = -inf) weigh nothing, also when they are all of the workgroup's rows
+      const double eu = (live && uv > -INFINITY) ? double(expf(float(uv - mu))) : 0.0;
+      double sf = live ? double(expf(float(fv - mf))) : 0.0, su = eu, tu = eu > 0.0 ? eu * fv : 0.0;
       for (int o = 1; o < 64; o <<= 1)
       {
         sf += __shfl_xor(sf, o, 64);

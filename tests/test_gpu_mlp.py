@@ -74,6 +74,29 @@ def test_mlp_gradients_match_oracle(gpu_ctx):
         net.close()
 
 
+def test_mlp_gradients_with_infinite_energies(gpu_ctx):
+    """Rows of infinite energy (overlaps) weigh nothing: a whole workgroup's 16 rows and one single row at +inf give
+    finite gradients and loss equal to the oracle's."""
+    print("case: mlp gradients, infinite energies", flush=True)
+    w, frac, energy, beta = problem(256, 5)
+    energy = energy.copy()
+    energy[32:48] = np.inf
+    energy[100] = np.inf
+    net = ra.Mlp(gpu_ctx)
+    net.set_params(w)
+    g, loss = net.gradients(frac, energy, beta)
+    le, ge = om.loss_and_grad(w, frac, energy, beta, emulate=True)
+    assert np.isfinite(loss) and np.all(np.isfinite(g))
+    assert abs(loss - le) <= 1e-4 * abs(le), (loss, le)
+    for name, (o, s) in om.OFFSETS.items():
+        if name == "b4":
+            continue
+        c = int(np.prod(s))
+        a, b = g[o:o + c], ge[o:o + c]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 0.02, name
+    net.close()
+
+
 def test_mlp_proposal_matches_oracle(gpu_ctx):
     print("case: mlp proposal", flush=True)
     w, _, _, _ = problem(64, 4)
