@@ -582,12 +582,11 @@ __global__ void k_zero(float* p, int n)
 
 // Adam (Kingma & Ba, 2015) on the FP32 masters, then the BF16 copies used by the products (row-major and, for
 // W1..W3, transposed); the gradient is cleared for the next minibatch's atomic sums.
-__global__ void k_adam(int n, float* __restrict__ w, float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, __bf16* __restrict__ wb, __bf16* __restrict__ wt, float lr, float b1,
-                       float b2, float eps, float c1, float c2)
+// One Adam update of parameter i (the gradient is cleared for the next step); returns the new weight.
+__device__ __forceinline__ float adamOne(size_t i, float* __restrict__ w, float* __restrict__ g, float* __restrict__ m,
+                                        float* __restrict__ v, __bf16* __restrict__ wb, float lr, float b1, float b2,
+                                        float eps, float c1, float c2)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   const float gi = g[i];
   g[i] = 0.f;
   const float mi = b1 * m[i] + (1.f - b1) * gi;
@@ -597,8 +596,62 @@ __global__ void k_adam(int n, float* __restrict__ w, float* __restrict__ g, floa
   const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   w[i] = wi;
   wb[i] = (__bf16)wi;
-  const long long ti = transposedIndex(size_t(i));
-  if (ti >= 0) wt[ti] = (__bf16)wi;
+  return wi;
+}
+
+// Adam over all parameters, with the BF16 copies refreshed: workgroups 0 .. ADAM_TILES - 1 take 32 x 32 tiles of
+// W1, W2 and W3 (rows k, columns j; coalesced along j) and write the transposed copy through an LDS tile (coalesced
+// along k); the remaining workgroups take the biases, w4 and b4 one parameter per thread.
+constexpr int ADAM_T1 = (NIN / 32) * (WIDTH / 32), ADAM_T2 = (WIDTH / 32) * (WIDTH / 32);
+constexpr int ADAM_TILES = ADAM_T1 + 2 * ADAM_T2;
+constexpr int ADAM_REST = int(OFF_W2 - OFF_B1) + int(OFF_W3 - OFF_B2) + int(NPARAM - OFF_B3);
+constexpr int ADAM_BLOCKS = ADAM_TILES + (ADAM_REST + 255) / 256;
+static_assert(NIN % 32 == 0 && WIDTH % 32 == 0, "Adam tiles are 32 x 32");
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ w, float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ v, __bf16* __restrict__ wb, __bf16* __restrict__ wt,
+                                              float lr, float b1, float b2, float eps, float c1, float c2)
+{
+  __shared__ __bf16 tile[32][34];
+  const int t = threadIdx.x;
+  const int b = blockIdx.x;
+  if (b < ADAM_TILES)
+  {
+    size_t off, toff;
+    int K, tb;
+    if (b < ADAM_T1)
+    {
+      off = OFF_W1, toff = OFF_T1, K = NIN, tb = b;
+    }
+    else if (b < ADAM_T1 + ADAM_T2)
+    {
+      off = OFF_W2, toff = OFF_T2, K = WIDTH, tb = b - ADAM_T1;
+    }
+    else
+    {
+      off = OFF_W3, toff = OFF_T3, K = WIDTH, tb = b - ADAM_T1 - ADAM_T2;
+    }
+    const int k0 = (tb / (WIDTH / 32)) * 32, j0 = (tb % (WIDTH / 32)) * 32;
+    const int tx = t & 31, ty = t >> 5;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+      const int k = k0 + ty + 8 * q;
+      tile[tx][ty + 8 * q] = (__bf16)adamOne(off + size_t(k) * WIDTH + j0 + tx, w, g, m, v, wb, lr, b1, b2, eps, c1, c2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+      const int j = j0 + ty + 8 * q;
+      wt[toff + size_t(j) * K + k0 + tx] = tile[ty + 8 * q][tx];
+    }
+    return;
+  }
+  const int e = (b - ADAM_TILES) * 256 + t;
+  if (e >= ADAM_REST) return;
+  const int nb1 = int(OFF_W2 - OFF_B1), nb2 = int(OFF_W3 - OFF_B2);
+  const size_t i = e < nb1 ? OFF_B1 + e : (e < nb1 + nb2 ? OFF_B2 + (e - nb1) : OFF_B3 + (e - nb1 - nb2));
+  adamOne(i, w, g, m, v, wb, lr, b1, b2, eps, c1, c2);
 }
 
 __global__ void k_zero_d(double* p, int n)
@@ -935,8 +988,8 @@ rg_status rg_mlp_train(rg_ctx* ctx, rg_mlp* p, int n, const double* frac, const 
         break;
       p->adamStep += 1;
       const float c1 = 1.f - std::pow(b1, float(p->adamStep)), c2 = 1.f - std::pow(b2, float(p->adamStep));
-      hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, ctx->stream, int(NPARAM), p->w, p->g, p->m,
-                         p->v, p->wb, p->wt, float(lr), b1, b2, eps, c1, c2);
+      hipLaunchKernelGGL(k_adam, dim3(ADAM_BLOCKS), dim3(256), 0, ctx->stream, p->w, p->g, p->m, p->v, p->wb, p->wt,
+                         float(lr), b1, b2, eps, c1, c2);
     }
     e = hipGetLastError();
     if (e == hipSuccess && st == RG_OK && ep + 1 < epochs)
