@@ -437,19 +437,18 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
 #pragma unroll
           for (int q = 0; q < NLJ; ++q)
           {
+            // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46.  No test of e4 != 0: a probe
+            // that does not interact with this type (e4 = 0, so c1 = c2 = c3 = 0 and the shift U(rc) = 0) adds exact
+            // zeros, and the staging keeps only images that interact with some probe.
             const LJPar& P = par[q][b.type];
-            if (P.e4 != 0.0)
-            {
-              // src/raspakit/potentials/interpolation/tricubic_derivative_lj.ixx:33-46
-              const double x = P.s2 * ir2;
-              const double t3 = x * x * x;
-              const double t6 = t3 * t3;
-              const double e = fma(P.e4, t6 - t3, -P.sh);
-              const double f1 = P.c1 * fma(2.0, t6, -t3) * ir2;
-              const double f2 = P.c2 * fma(7.0, t6, -2.0 * t3) * ir4;
-              const double f3 = P.c3 * fma(28.0, t6, -5.0 * t3) * ir6;
-              acc[q].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
-            }
+            const double x = P.s2 * ir2;
+            const double t3 = x * x * x;
+            const double t6 = t3 * t3;
+            const double e = fma(P.e4, t6 - t3, -P.sh);
+            const double f1 = P.c1 * fma(2.0, t6, -t3) * ir2;
+            const double f2 = P.c2 * fma(7.0, t6, -2.0 * t3) * ir4;
+            const double f3 = P.c3 * fma(28.0, t6, -5.0 * t3) * ir6;
+            acc[q].add(e, f1, f2, f3, u0, u1, u2, u01, u02, u12, u012);
           }
         }
         // a Coulomb-only launch stages charged images only.  Both precisions only form the radial factors; the one
