@@ -485,7 +485,8 @@ rg_status rg_mlp_set_params(rg_ctx* ctx, rg_mlp* mlp, const float* in);
 /* f(s) at n points (fractional coordinates, 3 per point). */
 rg_status rg_mlp_forward(rg_ctx* ctx, rg_mlp* mlp, int n, const double* frac, double* logdensity);
 /* Loss -sum_i p_i log softmax(f)_i with p = softmax(-beta U) over one batch (n a multiple of 64, <= 65536) and its
- * gradient with respect to every parameter; the parameters are not changed. */
+ * gradient with respect to every parameter; the parameters are not changed.  Points of energy +inf (overlaps) have
+ * p_i = 0, as long as some point of the batch has a finite energy. */
 rg_status rg_mlp_gradients(rg_ctx* ctx, rg_mlp* mlp, int n, const double* frac, const double* energy, double beta,
                            float* grads, double* loss);
 /* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over 'epochs' shuffled passes of minibatches of 'batch' points (a multiple
