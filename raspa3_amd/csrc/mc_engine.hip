@@ -1457,11 +1457,10 @@ __device__ __attribute__((always_inline)) void gcmcStep(const McRun& R, const Sy
   // src/raspakit/mc_moves/component/reinsertion.cpp:34-214) of a molecule of the component.  One growth call site
   // serves all three: an insertion grows, a deletion retraces molecule m, a reinsertion regrows m with its own atoms
   // left out of the background (pass 0) and then retraces its old configuration with the growth's unselected
-  // first-bead trials (pass 1).  Kernels with flexible chains (CHAIN) are built without the reinsertion: with it,
-  // hipcc (ROCm 7.2) fails on k_gcmc<true> with "Illegal instruction detected: Operand has incorrect register class"
-  // (a V_CMP on src_shared_base), so the host refuses p_reinsertion > 0 for them.
+  // first-bead trials (pass 1); a flexible chain is regrown and retraced bead by bead, its own atoms left out of
+  // the background in both passes.
   const double pRe = pRot + R.pReinsertion;
-  const bool reins = REINS && !CHAIN && uMove < pRe;
+  const bool reins = REINS && uMove < pRe;
   const bool insert = !reins && uMove < pRe + 0.5 * (1.0 - pRe);
   L.trials[reins ? 4 : (insert ? 0 : 1)] += 1;
   const int nc = L.ncomp[ci];
@@ -1833,5 +1832,6 @@ template __global__ void k_gcmc<false, false, true>(const McRun*, SysDev*, McSys
 template __global__ void k_gcmc<false, true, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
 template __global__ void k_gcmc<false, true, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
 template __global__ void k_gcmc<true, false, false>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
+template __global__ void k_gcmc<true, false, true>(const McRun*, SysDev*, McSys*, int, int, int, int, const McCtl*);
 
 }  // namespace rg

@@ -121,10 +121,6 @@ rg_status buildRun(rg_ctx* ctx, const rg_forcefield* ff, int ncomp, const rg_com
     C.eIntra = ff->use_charge ? self + excl : 0.0;
     C.index = c;
     C.flexible = d.flexible ? 1 : 0;
-    if (C.flexible && R.pReinsertion > 0.0)
-      return fail(ctx, RG_ERR_UNSUPPORTED,
-                  "component %d: the reinsertion move is built for rigid molecules only (runs with a flexible chain "
-                  "use the kernel without it)", c);
     if (C.flexible)
     {
       if (d.first_bead != 0 || C.charged || !(d.bond_length > 0.0) || d.bend_k_K < 0.0)
@@ -398,9 +394,9 @@ rg_status rg_mc_run(rg_ctx* ctx, rg_mc* mc, int cycles, int sample, int use_grap
   {
     const int nw = kNw[cand], wpb = kWpb[cand];
     if (bal) hipLaunchKernelGGL(k_mc_budget, dim3(1), dim3(256), 0, ctx->stream, mc->dRun, mc->dY, nsys, mc->dCtl, sample);
-    // kernel instances: chains (no balancing, no reinsertion) or rigid molecules with or without either
+    // kernel instances: chains (no balancing) or rigid molecules, each with or without the reinsertion move
     const bool reins = mc->run.pReinsertion > 0.0;
-    auto kernel = chain  ? k_gcmc<true, false, false>
+    auto kernel = chain  ? (reins ? k_gcmc<true, false, true> : k_gcmc<true, false, false>)
                   : bal  ? (reins ? k_gcmc<false, true, true> : k_gcmc<false, true, false>)
                          : (reins ? k_gcmc<false, false, true> : k_gcmc<false, false, false>);
     hipLaunchKernelGGL(kernel, dim3((nsys + wpb - 1) / wpb), dim3(64 * nw * wpb), nw * wpb * sizeof(WaveScratch),

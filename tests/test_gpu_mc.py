@@ -293,14 +293,30 @@ def test_gcmc_reinsertion_replays_oracle_trajectory(gpu_ctx):
     mc.close()
 
 
-def test_reinsertion_of_flexible_chains_is_refused(gpu_ctx):
-    """Kernels with flexible chains are built without the reinsertion move: p_reinsertion > 0 with a chain component
-    is refused on the host instead of being ignored."""
-    spec = w.forcefield_hexane_zeolite()
-    ff = ra.ForceField(gpu_ctx, spec)
-    s = ra.System(gpu_ctx, ff, np.diag([30.0] * 3), True, kmax=(0, 0, 0), capacity=8 * 256)
-    with pytest.raises(ra.RGError, match="reinsertion"):
-        ra.Mc(gpu_ctx, [s], [w.hexane_component(spec)], w.mc_params(p_reinsertion=0.1))
+def test_gcmc_hexane_reinsertion_replays_oracle_trajectory(gpu_ctx):
+    """CBMC reinsertion of flexible chains (MC_Moves::reinsertionMove, src/raspakit/mc_moves/component/reinsertion.cpp
+    :34-214): two n-hexane systems in ITQ-29 with 40 % reinsertion moves, each chain regrown bead by bead with its own
+    atoms left out and its old configuration retraced with Esselink's stored first-bead weight, replay the oracle
+    move by move for 4 + 12 cycles; the running energy equals the recomputed total."""
+    print("case: gcmc hexane reinsertion replay", flush=True)
+    spec, fw, unit, ff, f, dims, grids, host = hexane_itq29(gpu_ctx)
+    comp = w.hexane_component(spec, 0.3)
+    params = w.mc_params(pressure_pa=10.0, seed=17, max_molecules=64, production_cycles=12, p_translation=0.15,
+                         p_rotation=0.15, p_reinsertion=0.4)
+    systems = [ra.System(gpu_ctx, ff, fw.cell, True, framework=f, grids=grids, kmax=(0, 0, 0), capacity=8 * 64)
+               for _ in range(2)]
+    mc = ra.Mc(gpu_ctx, systems, [comp], params)
+    mc.run(4, sample=False, graph=False)
+    mc.run(12, sample=True, graph=True)
+    for sidx in range(2):
+        g = compare_with_oracle(mc, sidx, spec, fw, [comp], params, (0, 0, 0), host, None, dims, unit, (4, 12))
+        assert g["n_molecules"] > 0 and g["trials"][4] > 0
+    assert sum(mc.state(s)["accepted"][4] for s in range(2)) > 0
+    tot = mc.total_energy()
+    for sidx in range(2):
+        e = mc.state(sidx)["energy"]
+        assert abs(tot[sidx] - e) <= 1e-7 * max(1.0, abs(e)), (tot[sidx], e)
+    mc.close()
 
 
 def test_gcmc_mixture_with_proposal_replays_oracle(gpu_ctx):
