@@ -158,13 +158,13 @@ typedef enum rg_grid_precision
 {
   RG_GRID_FP64 = 0,      /* every pair term in FP64 (what rg_grid_build_many does) */
   RG_GRID_SPLIT_FP32 = 1 /* framework images at least split_radius from every point of a 64-point block: radial
-                          * factors in FP32 (hardware rsq and exp2, a fitted FP32 erfcx polynomial), accumulated in
+                          * factors in FP32 (hardware rsq and exp2, erfc from a fitted FP32 polynomial), accumulated in
                           * FP64; nearer images in FP64.  Lennard-Jones grids and triquintic grids stay FP64. */
 } rg_grid_precision;
 
 /* System::createFrameworkInterpolationGrids (src/raspakit/system_interpolation_grids.cpp:176-238) as
  * rg_grid_build_many, with the precision of the Ewald real-space grid chosen; async != 0 returns after the launch.
- * RG_ERR_UNSUPPORTED when the FP32 erfcx polynomial cannot reach 4e-8 relative on [alpha split_radius, alpha rc];
+ * RG_ERR_UNSUPPORTED when the FP32 erfc polynomial cannot reach 4e-8 relative on [alpha split_radius, alpha rc];
  * RG_ERR_INVALID for a split_radius below sqrt(0.1) A (inside the reference's clamp of r^2 at 0.1 A^2). */
 rg_status rg_grid_build_many_precision(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework* fw, int n_grids,
                                        const rg_grid_type* types, const int* probes, const double* cutoffs,

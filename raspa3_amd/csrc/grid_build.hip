@@ -54,7 +54,7 @@ constexpr int MAXG = 4;    // LJ grids per launch
 constexpr int MAXT = 16;   // distinct framework pseudo-atom types
 constexpr int CH = 256;    // candidates per LDS chunk
 constexpr int MAXROWS = 1024;
-constexpr int kSplitDeg = 9;  // degree of the FP32 erfcx polynomial of the split-precision Ewald grid
+constexpr int kSplitDeg = 6;  // degree of the FP32 erfc polynomial of the split-precision Ewald grid
 
 struct GridArgs
 {
@@ -79,7 +79,7 @@ struct GridArgs
   double overlap;
   double* out[MAXG + 1];
   // split precision of the Ewald real-space grid: images at least sqrt(rsplit2) from every point of a wavefront take
-  // FP32 radial factors (erfcx from a polynomial in t = r xs + xo, |t| <= 1)
+  // FP32 radial factors (x erfcx(x) / alpha at x = alpha r from a polynomial in t = xs / r + xo, |t| <= 1)
   double rsplit2;
   double a2l2e;  // alpha^2 / ln 2
   float xs32, xo32, k2a32, alpha2_32;
@@ -460,12 +460,14 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
           if (SPLIT && !__ballot(rr < A.rsplit2))
           {
             // every lane of the wavefront is at least the split radius from this image: the radial factors in FP32
-            // (hardware rsq and exp2, erfcx from the fitted polynomial).  The exponent -alpha^2 r^2 / ln 2 = k + f
-            // (k = its floor) is reduced in FP64 so that only f in [0, 1) passes through FP32.
+            // (hardware rsq and exp2, x erfcx(x) / alpha at x = alpha r from the polynomial in 1/r).  The exponent
+            // -alpha^2 r^2 / ln 2 = k + f (k = its floor) is reduced in FP64 so that only f in [0, 1) passes
+            // through FP32.  With ecr = q erfc(alpha r) r and E = q 2 alpha / sqrt(pi) r^2 exp(-alpha^2 r^2), the
+            // factors of the FP64 branch below become e = ecr / r^2, f1 = -(E + ecr) / r^4,
+            // f2 = (2 E alpha^2 r^2 + 3 (E + ecr)) / r^6, f3 = -(E alpha^2 r^2 (4 alpha^2 r^2 + 10) + 15 (E + ecr)) / r^8.
             const float rr32 = static_cast<float>(rr);
             const float ir = __builtin_amdgcn_rsqf(rr32);
-            const float r = rr32 * ir;
-            const float t = fmaf(r, A.xs32, A.xo32);
+            const float t = fmaf(ir, A.xs32, A.xo32);
             float pe = A.ecx[kSplitDeg];
 #pragma unroll
             for (int j = kSplitDeg - 1; j >= 0; --j) pe = fmaf(pe, t, A.ecx[j]);
@@ -474,17 +476,19 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
             const float ex = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(y - k)),
                                                      static_cast<int>(k));
             const float qex = b.qf * ex;                       // q exp(-alpha^2 r^2)
-            const float ec = qex * pe;                         // q erfc(alpha r)
-            const float etr = A.k2a32 * r * qex;
+            const float ecr = qex * pe;                        // q erfc(alpha r) r
+            const float E = A.k2a32 * qex * rr32;
             const float ar2 = A.alpha2_32 * rr32;
+            const float s = E + ecr;
+            const float ear = E * ar2;
             const float ir2 = ir * ir;
-            const float ir3 = ir2 * ir;
-            const float ir5 = ir3 * ir2;
-            const float ir7 = ir5 * ir2;
-            e = double(ec * ir);
-            f1 = double(-(etr + ec) * ir3);
-            f2 = double(fmaf(etr, fmaf(2.0f, ar2, 3.0f), 3.0f * ec) * ir5);
-            f3 = double(-fmaf(etr, fmaf(ar2, fmaf(4.0f, ar2, 10.0f), 15.0f), 15.0f * ec) * ir7);
+            const float ir4 = ir2 * ir2;
+            const float ir6 = ir4 * ir2;
+            const float ir8 = ir4 * ir4;
+            e = double(ecr * ir2);
+            f1 = double(-s * ir4);
+            f2 = double(fmaf(2.0f, ear, 3.0f * s) * ir6);
+            f3 = double(-fmaf(ear, fmaf(4.0f, ar2, 10.0f), 15.0f * s) * ir8);
           }
           else
           {
@@ -830,19 +834,22 @@ LJPar ljPar(const rg_forcefield* ff, int probe, int type)
   return P;
 }
 
-// erfcx(x) = exp(x^2) erfc(x) on [x0, x1] as a polynomial of degree kSplitDeg in t = x * xs + xo (|t| <= 1): the
-// Chebyshev interpolant at kSplitDeg + 1 Chebyshev points, in monomial form, with FP32 coefficients.  Returns the
-// largest relative error of the FP32-coefficient polynomial (evaluated in FP64) against the FP64 library erfc over
-// 4001 points of the interval.
-double fitErfcxF32(double x0, double x1, float* coef, double& xs, double& xo)
+// g(x) = x erfcx(x) / alpha = x exp(x^2) erfc(x) / alpha for x = alpha r in [x0, x1], as a polynomial of degree
+// kSplitDeg in t = u * us + uo (|t| <= 1) of u = 1 / x: the Chebyshev interpolant at kSplitDeg + 1 Chebyshev points
+// of u, in monomial form, with FP32 coefficients.  x erfcx(x) tends to 1 / sqrt(pi) with a series in 1 / x^2, so in
+// u it needs degree 6 where erfcx itself, linear in x, needs 9 for the same FP32 accuracy.  Returns the largest
+// relative error of the FP32-coefficient polynomial (evaluated in FP64) against the FP64 library erfc over 4001
+// points of the interval.
+double fitErfcxF32(double x0, double x1, double alpha, float* coef, double& us, double& uo)
 {
   constexpr int N = kSplitDeg + 1;
-  const double mid = 0.5 * (x0 + x1), half = 0.5 * (x1 - x0);
-  xs = 1.0 / half;
-  xo = -mid / half;
-  auto erfcx = [](double x) { return std::exp(x * x) * std::erfc(x); };
+  const double u0 = 1.0 / x1, u1 = 1.0 / x0;
+  const double mid = 0.5 * (u0 + u1), half = 0.5 * (u1 - u0);
+  us = 1.0 / half;
+  uo = -mid / half;
+  auto g = [alpha](double x) { return x * std::exp(x * x) * std::erfc(x) / alpha; };
   double fval[N], cheb[N];
-  for (int k = 0; k < N; ++k) fval[k] = erfcx(mid + half * std::cos(M_PI * (k + 0.5) / N));
+  for (int k = 0; k < N; ++k) fval[k] = g(1.0 / (mid + half * std::cos(M_PI * (k + 0.5) / N)));
   for (int j = 0; j < N; ++j)
   {
     double sum = 0.0;
@@ -870,10 +877,10 @@ double fitErfcxF32(double x0, double x1, float* coef, double& xs, double& xo)
   for (int k = 0; k <= 4000; ++k)
   {
     const double x = x0 + (x1 - x0) * k / 4000.0;
-    const double t = x * xs + xo;
+    const double t = us / x + uo;
     double p = coef[N - 1];
     for (int i = N - 2; i >= 0; --i) p = p * t + coef[i];
-    worst = std::max(worst, std::fabs(p / erfcx(x) - 1.0));
+    worst = std::max(worst, std::fabs(p / g(x) - 1.0));
   }
   return worst;
 }
@@ -995,19 +1002,21 @@ rg_status gridBuildMany(rg_ctx* ctx, const rg_forcefield* ff, const rg_framework
   {
     // images within the split radius of a wavefront keep the FP64 path; the others have alpha r in
     // [alpha rsplit, alpha rc] (with a relative margin of 1e-4 for the FP32 rounding of r)
-    double xs = 0.0, xo = 0.0;
-    const double err = fitErfcxF32(ff->alpha * splitRadius * (1.0 - 1e-4), ff->alpha * rc * (1.0 + 1e-4), A.ecx, xs, xo);
+    double us = 0.0, uo = 0.0;
+    const double err = fitErfcxF32(ff->alpha * splitRadius * (1.0 - 1e-4), ff->alpha * rc * (1.0 + 1e-4), ff->alpha,
+                                   A.ecx, us, uo);
     if (!(err < 4e-8))
       return rg::fail(ctx, RG_ERR_UNSUPPORTED,
-                      "rg_grid_build: the FP32 erfcx polynomial reaches only %.2e relative on alpha r in [%g, %g]; "
+                      "rg_grid_build: the FP32 erfc polynomial reaches only %.2e relative on alpha r in [%g, %g]; "
                       "choose a larger split radius", err, ff->alpha * splitRadius, ff->alpha * rc);
     if (splitRadius * splitRadius < 0.1)
       return rg::fail(ctx, RG_ERR_INVALID, "rg_grid_build: a split radius below sqrt(0.1) A lies inside the overlap "
                       "clamp of the pair distance");
     A.rsplit2 = splitRadius * splitRadius;
     A.a2l2e = ff->alpha * ff->alpha / M_LN2;
-    A.xs32 = static_cast<float>(ff->alpha * xs);
-    A.xo32 = static_cast<float>(xo);
+    // t = u us + uo with u = 1 / (alpha r): the kernel forms t from its FP32 1 / r
+    A.xs32 = static_cast<float>(us / ff->alpha);
+    A.xo32 = static_cast<float>(uo);
     A.k2a32 = static_cast<float>(A.k2a);
     A.alpha2_32 = static_cast<float>(A.alpha2);
   }
