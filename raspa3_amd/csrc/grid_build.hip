@@ -396,6 +396,15 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
   Acc<TRIC> acc[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) acc[g].zero();
+  // the split path's polynomial offset and leading coefficient, held in vector registers: an FMA reads at most one
+  // scalar operand, and with both of a pair in scalar registers the compiler re-copied one into a vector register
+  // for every pair
+  float xo32 = A.xo32, lead32 = A.ecx[kSplitDeg];
+  if (SPLIT)
+  {
+    asm volatile("" : "+v"(xo32));
+    asm volatile("" : "+v"(lead32));
+  }
 
   for (int base = 0; base < total; base += 256)
   {
@@ -417,11 +426,17 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
         // host's rsplit^2 >= 0.1 keeps above the clamp
         const double rrc = fmax(rr, 0.1);
         double u0, u1, u2;
+        float qf = 0.0f;  // the staged FP32 charge (triclinic: read with u2 in one 16-byte LDS read)
         if (TRIC)
         {
-          u0 = T.pu0 - b.u0;
-          u1 = T.pu1 - b.u1;
-          u2 = T.pu2 - b.u2;
+          const double2 c2 = cb[4 * c + 2], c3 = cb[4 * c + 3];  // (u0, u1), (u2, type | qf)
+          u0 = T.pu0 - c2.x;
+          u1 = T.pu1 - c2.y;
+          u2 = T.pu2 - c3.x;
+          qf = __uint_as_float(static_cast<unsigned>(static_cast<unsigned long long>(__double_as_longlong(c3.y)) >> 32));
+          // formed here, qf keeps the record's last 16 bytes one LDS read; left to the FP32 branch, the compiler
+          // split off a 4-byte read (and its address copy) there
+          if (SPLIT) asm volatile("" : "+v"(qf));
         }
         else
         {
@@ -467,15 +482,15 @@ __global__ __launch_bounds__(256) void k_grid_build(GridArgs A)
             // f2 = (2 E alpha^2 r^2 + 3 (E + ecr)) / r^6, f3 = -(E alpha^2 r^2 (4 alpha^2 r^2 + 10) + 15 (E + ecr)) / r^8.
             const float rr32 = static_cast<float>(rr);
             const float ir = __builtin_amdgcn_rsqf(rr32);
-            const float t = fmaf(ir, A.xs32, A.xo32);
-            float pe = A.ecx[kSplitDeg];
+            const float t = fmaf(ir, A.xs32, xo32);
+            float pe = lead32;
 #pragma unroll
             for (int j = kSplitDeg - 1; j >= 0; --j) pe = fmaf(pe, t, A.ecx[j]);
             const double y = rr * -A.a2l2e;
             const double k = floor(y);
             const float ex = __builtin_amdgcn_ldexpf(__builtin_amdgcn_exp2f(static_cast<float>(y - k)),
                                                      static_cast<int>(k));
-            const float qex = b.qf * ex;                       // q exp(-alpha^2 r^2)
+            const float qex = (TRIC ? qf : b.qf) * ex;         // q exp(-alpha^2 r^2)
             const float ecr = qex * pe;                        // q erfc(alpha r) r
             const float E = A.k2a32 * qex * rr32;
             const float ar2 = A.alpha2_32 * rr32;
