@@ -8,7 +8,11 @@ since the proposal's density enters the acceptance rule exactly.  The loading is
 own production means, which are independent (sigma = their standard deviation / sqrt(32), within 3 sigma: the five
 pooled blocks of the reference's estimator understate the scatter between runs at this loading); the isosteric heat
 through the reference's estimators (rg_enthalpy_of_adsorption on the production blocks pooled over the systems,
-sigma = 95 % interval / t_4), within 2 sigma.
+sigma = 95 % interval / t_4), within 3 sigma.  Each sigma is a five-block estimate (4 degrees of freedom), so the
+difference of the two heats over its combined sigma follows roughly Student's t with 8 degrees of freedom: a 2-sigma
+bar fails about 8 % of correct runs, a 3-sigma bar about 2 %.  The learned run is not the same draw on every run (the
+network's weight gradients are summed with FP32 atomics, so training is not bit-reproducible), so the bar has to hold
+for fresh draws.
 """
 import numpy as np
 import pytest
@@ -98,4 +102,4 @@ def test_learned_proposal_raises_acceptance_at_matched_loading(gpu_ctx):
           f"{learned['loading_sigma']:.4f}, Qst {learned['qst']:.3f} +- {learned['qst_sigma']:.3f} kJ/mol", flush=True)
     assert learned["acceptance"] > 1.3 * plain["acceptance"]
     assert abs(learned["loading"] - plain["loading"]) <= 3.0 * np.hypot(learned["loading_sigma"], plain["loading_sigma"])
-    assert abs(learned["qst"] - plain["qst"]) <= 2.0 * np.hypot(learned["qst_sigma"], plain["qst_sigma"])
+    assert abs(learned["qst"] - plain["qst"]) <= 3.0 * np.hypot(learned["qst_sigma"], plain["qst_sigma"])
