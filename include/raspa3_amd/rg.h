@@ -304,9 +304,10 @@ rg_status rg_ewald_accept(rg_ctx* ctx, rg_system* sys);
 rg_status rg_ewald_energy(rg_ctx* ctx, const rg_system* sys, double* out);
 
 /* ---------------------------------------------------------------------------------------------
- * Monte Carlo on the device: Widom insertions and multi-system GCMC of rigid molecules
- * (src/raspakit/mc_moves/component/{widom,insertion_cbmc,deletion_cbmc}.cpp; first beads
- * src/raspakit/cbmc/multiple_first_bead.cpp; orientations of the rigid body src/raspakit/cbmc/operators.cpp:958-968).
+ * Monte Carlo on the device: Widom insertions and multi-system GCMC of rigid molecules and flexible linear chains
+ * (src/raspakit/mc_moves/component/{widom,insertion_cbmc,deletion_cbmc,translation,rotation,reinsertion}.cpp; first
+ * beads src/raspakit/cbmc/multiple_first_bead.cpp; orientations of the rigid body
+ * src/raspakit/cbmc/operators.cpp:958-968; chain growth src/raspakit/cbmc/flexible_molecules/insertion.cpp).
  * ------------------------------------------------------------------------------------------- */
 
 /* A component (Component, src/raspakit/component.ixx).  Rigid (flexible = 0): up to 4 atoms in a reference geometry;
